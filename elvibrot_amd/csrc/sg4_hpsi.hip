@@ -1,0 +1,884 @@
+// MI355X (gfx950) Smolyak SG4 H.psi: device plan, kernels and C ABI.
+//
+// Counterpart of sub_TabOpPsi_FOR_SGtype4 (SRC/sub_Operator/sub_OpPsi_SG4.f90
+// :681-977) and of the per-term action sub_TabOpPsi_OF_ONEDP_FOR_SGtype4
+// (:1352-1666).  The reference loops over Smolyak terms with OpenMP, gathers
+// each term's coefficients, transforms basis->grid mode by mode, applies the
+// potential and the kinetic derivatives on the grid, transforms back and
+// scatter-adds W_t times the result.  Here one workgroup handles one
+// (term, chunk of vectors) work item entirely in LDS:
+//   pass BG0   : gather from the packed block fused with the first mode product
+//   passes BG  : remaining basis->grid mode products (LDS -> LDS)
+//   passes GR  : V psi + sum_k c_k d2_k psi on the grid
+//   passes GB  : grid->basis mode products, the last one fused with the
+//                weighted scatter-add into the packed result
+// Each mode product is done fibre by fibre: one thread loads a fibre into
+// registers and multiplies it by the small 1-D matrix, whose entries are
+// wave-uniform.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sg4_internal.h"
+
+namespace sg4 {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+#define HIP_TRY(expr)                                                             \
+  do {                                                                            \
+    hipError_t _e = (expr);                                                       \
+    if (_e != hipSuccess) {                                                       \
+      set_error(std::string("HIP error: ") + hipGetErrorString(_e) + " at " #expr); \
+      return SG4_ERR_HIP;                                                         \
+    }                                                                             \
+  } while (0)
+
+constexpr int MAXACT = 16;
+constexpr int NT = 256;                 // threads per workgroup
+constexpr int LDS_SMALL = 64 * 1024;    // LDS budget of the common class (2 WG/CU)
+constexpr int LDS_MAX = 160 * 1024;     // gfx950 LDS per workgroup
+
+struct MatDev {          // offsets (in doubles) of the 1-D matrices of (k,l)
+  int64_t B, BtW, D1, D2;
+  int32_t nq, nb;
+  double b00, btw00, d200;  // entries used when nq = nb = 1
+};
+
+struct TermDev {
+  int64_t boff, qoff;
+  double wgt;      // W_t times the 1x1 transforms of the inactive modes
+  double vshift;   // type 1: sum over inactive modes of c_k d2_k(0,0)
+  int32_t nb, nq;  // product sizes
+  int32_t nact, pad;
+  int16_t mat[MAXACT];   // MatDev index of each active mode
+  int8_t kmode[MAXACT];  // mode number of each active mode
+};
+
+struct Item {
+  int32_t term;
+  int16_t v0;
+  int8_t lgnv;
+  int8_t big;     // second buffer in global scratch (slot) instead of LDS
+  int32_t slot;
+  int32_t pad;
+};
+
+struct KArgs {
+  const Item* items;
+  const TermDev* terms;
+  const MatDev* md;
+  const double* mats;
+  const int32_t* map;
+  const double* V;
+  const double* ck;
+  const double* psi;
+  double* hpsi;
+  double* scratch;
+  int64_t slot_size;
+  int nvec;
+};
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int fdiv(int n, int d, float inv) {
+  int q = (int)((float)n * inv);
+  int r = n - q * d;
+  while (r < 0) { --q; r += d; }
+  while (r >= d) { ++q; r -= d; }
+  return q;
+}
+
+enum { IN_LDS = 0, IN_GATHER = 1 };
+enum { OUT_STORE = 0, OUT_ADD = 1, OUT_VSTORE = 2, OUT_SCATTER = 3 };
+
+struct PassCtx {
+  const double* __restrict__ M;  // NO x NI, column-major
+  const double* in;
+  double* out;
+  int AV;        // A * nv
+  float invAV;
+  int A;
+  int nfib;
+  int lgnv;
+  double coef;
+  // gather / scatter
+  const int32_t* __restrict__ map;  // term's map slice
+  const double* __restrict__ psi;
+  double* hpsi;
+  int nvec, v0;
+  double wgt;
+  // grid potential
+  const double* __restrict__ V;  // term's grid slice
+  double vshift;
+};
+
+template <int NI, int NO, int KIN, int KOUT>
+__device__ __forceinline__ void mode_pass(const PassCtx& c) {
+  for (int f = threadIdx.x; f < c.nfib; f += NT) {
+    const int b = fdiv(f, c.AV, c.invAV);
+    const int av = f - b * c.AV;
+    double x[NI];
+    if constexpr (KIN == IN_GATHER) {
+      // first active mode: A = 1 so av is the vector index
+      const int32_t* mp = c.map + NI * b;
+#pragma unroll
+      for (int j = 0; j < NI; ++j) {
+        const int m = mp[j];
+        x[j] = (m >= 0) ? c.psi[(int64_t)m * c.nvec + c.v0 + av] : 0.0;
+      }
+    } else {
+      const double* src = c.in + av + c.AV * NI * b;
+#pragma unroll
+      for (int j = 0; j < NI; ++j) x[j] = src[c.AV * j];
+    }
+#pragma unroll
+    for (int i = 0; i < NO; ++i) {
+      double y = 0.0;
+#pragma unroll
+      for (int j = 0; j < NI; ++j) y = fma(c.M[i + NO * j], x[j], y);
+      if constexpr (KOUT == OUT_STORE) {
+        c.out[av + c.AV * (i + NO * b)] = y;
+      } else if constexpr (KOUT == OUT_ADD) {
+        double* o = c.out + av + c.AV * (i + NO * b);
+        *o = fma(c.coef, y, *o);
+      } else if constexpr (KOUT == OUT_VSTORE) {
+        const int e = (av >> c.lgnv) + c.A * (i + NO * b);
+        c.out[av + c.AV * (i + NO * b)] = fma(c.coef, y, (c.V[e] + c.vshift) * x[i]);
+      } else {
+        // last active mode: nothing after it, b == 0
+        const int e = (av >> c.lgnv) + c.A * i;
+        const int m = c.map[e];
+        if (m >= 0)
+          atomicAdd(&c.hpsi[(int64_t)m * c.nvec + c.v0 + (av & ((1 << c.lgnv) - 1))], c.wgt * y);
+      }
+    }
+  }
+}
+
+// generic sizes (rectangular or > 24): runtime loops
+template <int KIN, int KOUT>
+__device__ void mode_pass_generic(const PassCtx& c, int NI, int NO) {
+  for (int f = threadIdx.x; f < c.nfib; f += NT) {
+    const int b = fdiv(f, c.AV, c.invAV);
+    const int av = f - b * c.AV;
+    double x[SG4_MAX_N];
+    for (int j = 0; j < NI; ++j) {
+      if constexpr (KIN == IN_GATHER) {
+        const int m = c.map[NI * b + j];
+        x[j] = (m >= 0) ? c.psi[(int64_t)m * c.nvec + c.v0 + av] : 0.0;
+      } else {
+        x[j] = c.in[av + c.AV * (j + NI * b)];
+      }
+    }
+    for (int i = 0; i < NO; ++i) {
+      double y = 0.0;
+      for (int j = 0; j < NI; ++j) y = fma(c.M[i + NO * j], x[j], y);
+      if constexpr (KOUT == OUT_STORE) {
+        c.out[av + c.AV * (i + NO * b)] = y;
+      } else if constexpr (KOUT == OUT_ADD) {
+        double* o = c.out + av + c.AV * (i + NO * b);
+        *o = fma(c.coef, y, *o);
+      } else if constexpr (KOUT == OUT_VSTORE) {
+        const int e = (av >> c.lgnv) + c.A * (i + NO * b);
+        c.out[av + c.AV * (i + NO * b)] = fma(c.coef, y, (c.V[e] + c.vshift) * x[i]);
+      } else {
+        const int e = (av >> c.lgnv) + c.A * i;
+        const int m = c.map[e];
+        if (m >= 0)
+          atomicAdd(&c.hpsi[(int64_t)m * c.nvec + c.v0 + (av & ((1 << c.lgnv) - 1))], c.wgt * y);
+      }
+    }
+  }
+}
+
+template <int KIN, int KOUT>
+__device__ __forceinline__ void dispatch_pass(const PassCtx& c, int NI, int NO) {
+  if (NI == NO) {
+    switch (NI) {
+#define SG4_CASE(n) case n: mode_pass<n, n, KIN, KOUT>(c); return;
+      SG4_CASE(2) SG4_CASE(3) SG4_CASE(4) SG4_CASE(5) SG4_CASE(6) SG4_CASE(7)
+      SG4_CASE(8) SG4_CASE(9) SG4_CASE(10) SG4_CASE(11) SG4_CASE(12) SG4_CASE(13)
+      SG4_CASE(14) SG4_CASE(15) SG4_CASE(16) SG4_CASE(17) SG4_CASE(18) SG4_CASE(19)
+      SG4_CASE(20) SG4_CASE(21) SG4_CASE(22) SG4_CASE(23) SG4_CASE(24)
+#undef SG4_CASE
+      default: break;
+    }
+  }
+  mode_pass_generic<KIN, KOUT>(c, NI, NO);
+}
+
+// ---------------------------------------------------------------------------
+// Type-1 operator, FP64: one workgroup per work item.
+// ---------------------------------------------------------------------------
+template <bool BIG>
+__device__ __forceinline__ void term_t1_f64(const KArgs& a, const Item& it, double* lds) {
+  const TermDev* T = &a.terms[it.term];
+  const int lg = it.lgnv, nv = 1 << lg;
+  const int nact = T->nact;
+  const int32_t* map = a.map + T->boff;
+  const double* V = a.V + T->qoff;
+
+  if (nact == 0) {  // the one-point term
+    if ((int)threadIdx.x < nv) {
+      const int m = map[0];
+      if (m >= 0) {
+        const int64_t r = (int64_t)m * a.nvec + it.v0 + threadIdx.x;
+        atomicAdd(&a.hpsi[r], T->wgt * (V[0] + T->vshift) * a.psi[r]);
+      }
+    }
+    return;
+  }
+
+  int nq[MAXACT], nb[MAXACT];
+  const MatDev* md[MAXACT];
+  for (int p = 0; p < nact; ++p) {
+    md[p] = &a.md[T->mat[p]];
+    nq[p] = md[p]->nq;
+    nb[p] = md[p]->nb;
+  }
+  double* buf0 = lds;
+  double* buf1 = BIG ? a.scratch + (int64_t)it.slot * a.slot_size : lds + (size_t)T->nq * nv;
+
+  PassCtx c;
+  c.map = map; c.psi = a.psi; c.hpsi = a.hpsi; c.nvec = a.nvec; c.v0 = it.v0;
+  c.wgt = T->wgt; c.V = V; c.vshift = T->vshift; c.lgnv = lg; c.coef = 1.0;
+
+  // ---- basis -> grid ----
+  int A = 1;            // product of already transformed sizes
+  int Bafter = T->nb;   // product of untransformed sizes (incl. current)
+  double* cur = nullptr;
+  double* nxt = buf0;
+  for (int p = 0; p < nact; ++p) {
+    Bafter /= nb[p];
+    c.M = a.mats + md[p]->B;
+    c.in = cur; c.out = nxt;
+    c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
+    c.nfib = A * nv * Bafter;
+    if (p == 0) dispatch_pass<IN_GATHER, OUT_STORE>(c, nb[p], nq[p]);
+    else dispatch_pass<IN_LDS, OUT_STORE>(c, nb[p], nq[p]);
+    __syncthreads();
+    A *= nq[p];
+    cur = nxt;
+    nxt = (nxt == buf0) ? buf1 : buf0;
+  }
+  // ---- grid: V psi + sum_k c_k d2_k psi ----
+  double* pg = cur;
+  double* acc = nxt;
+  {
+    int Ab = 1, Aa = T->nq;
+    for (int p = 0; p < nact; ++p) {
+      Aa /= nq[p];
+      c.M = a.mats + md[p]->D2;
+      c.in = pg; c.out = acc;
+      c.A = Ab; c.AV = Ab * nv; c.invAV = 1.0f / (float)c.AV;
+      c.nfib = Ab * nv * Aa;
+      c.coef = a.ck[T->kmode[p]];
+      if (p == 0) dispatch_pass<IN_LDS, OUT_VSTORE>(c, nq[p], nq[p]);
+      else dispatch_pass<IN_LDS, OUT_ADD>(c, nq[p], nq[p]);
+      __syncthreads();
+      Ab *= nq[p];
+    }
+  }
+  // ---- grid -> basis, last pass scatters ----
+  c.coef = 1.0;
+  cur = acc;
+  nxt = pg;
+  A = 1;
+  int Aq = T->nq;
+  for (int p = 0; p < nact; ++p) {
+    Aq /= nq[p];
+    c.M = a.mats + md[p]->BtW;
+    c.in = cur; c.out = nxt;
+    c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
+    c.nfib = A * nv * Aq;
+    if (p == nact - 1) {
+      dispatch_pass<IN_LDS, OUT_SCATTER>(c, nq[p], nb[p]);
+    } else {
+      dispatch_pass<IN_LDS, OUT_STORE>(c, nq[p], nb[p]);
+      __syncthreads();
+    }
+    A *= nb[p];
+    double* t = cur; cur = nxt; nxt = t;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_hpsi_t1_f64(KArgs a) {
+  extern __shared__ double lds[];
+  const Item it = a.items[blockIdx.x];
+  if (it.big) term_t1_f64<true>(a, it, lds);
+  else term_t1_f64<false>(a, it, lds);
+}
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+__global__ void k_cols_to_rows(const double* __restrict__ in, double* __restrict__ out,
+                               int64_t nrow, int nvec) {
+  const int64_t n = nrow * nvec;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / nvec;
+    const int v = (int)(i - r * nvec);
+    out[i] = in[(int64_t)v * nrow + r];
+  }
+}
+__global__ void k_rows_to_cols(const double* __restrict__ in, double* __restrict__ out,
+                               int64_t nrow, int nvec) {
+  const int64_t n = nrow * nvec;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int v = (int)(i / nrow);
+    const int64_t r = i - (int64_t)v * nrow;
+    out[i] = in[r * nvec + v];
+  }
+}
+__global__ void k_fill(double* __restrict__ p, int64_t n, double val) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = val;
+}
+
+// Henon-Heiles grid of every term (calcN_op of
+// Working_tests/MPI_tests/6D_Davidson_S0/sub_system_HenonHeiles.f:14-43):
+// V = sum 1/2 q_i^2 + lambda sum (q_i^2 q_{i+1} - q_{i+1}^3/3)
+struct HHArgs {
+  const int32_t* levels;  // D x nterms
+  const int64_t* qoff;
+  const double* nodes;    // for k, for l: nq nodes
+  const int64_t* node_off;  // (k*(Lmax+1)+l) -> offset
+  const int32_t* nqtab;     // (k*(Lmax+1)+l) -> nq
+  int D, Lmax;
+  int64_t nterms;
+  double lambda;
+  double* V;
+};
+__global__ void k_grid_hh(HHArgs a) {
+  for (int64_t t = blockIdx.x; t < a.nterms; t += gridDim.x) {
+    const int64_t q0 = a.qoff[t], nqt = a.qoff[t + 1] - q0;
+    for (int64_t e = threadIdx.x; e < nqt; e += blockDim.x) {
+      int64_t r = e;
+      double v = 0.0, qprev = 0.0;
+      for (int k = 0; k < a.D; ++k) {
+        const int l = a.levels[(int64_t)t * a.D + k];
+        const int idx = k * (a.Lmax + 1) + l;
+        const int n = a.nqtab[idx];
+        const int i = (int)(r % n);
+        r /= n;
+        const double q = a.nodes[a.node_off[idx] + i];
+        v += 0.5 * q * q;
+        if (k > 0) v += a.lambda * (qprev * qprev * q - q * q * q / 3.0);
+        qprev = q;
+      }
+      a.V[q0 + e] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// plan
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  int alloc(size_t bytes) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+    n = bytes;
+    if (bytes == 0) return SG4_OK;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      set_error("device allocation of " + std::to_string(bytes) + " bytes failed");
+      p = nullptr;
+      return SG4_ERR_OOM;
+    }
+    return SG4_OK;
+  }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+struct ItemPlan {
+  DevBuf items, scratch;
+  int64_t nitems = 0, nbig = 0, slot_size = 0;
+  int lds = 0;
+};
+
+}  // namespace sg4
+
+struct sg4_plan {
+  int D = 0, Lmax = 0;
+  int64_t nterms = 0, nb_ba = 0, sum_nb = 0, sum_nq = 0;
+  int max_nvec = 0;
+  int type_op = 0;
+  hipStream_t stream = nullptr;
+  std::vector<int32_t> h_levels, h_nq, h_nb;
+  std::vector<int64_t> h_qoff, h_boff;
+  std::vector<double> h_weights;
+  std::vector<sg4::MatDev> h_md;
+  std::vector<sg4::TermDev> h_terms;
+  std::vector<double> h_ck;
+  bool have_nodes = false;
+  sg4::DevBuf d_mats, d_md, d_terms, d_map, d_V, d_ck, d_nodes, d_node_off, d_nqtab,
+      d_levels, d_qoff;
+  sg4::DevBuf d_rows_in, d_rows_out, d_cols_in, d_cols_out;
+  std::map<int, std::unique_ptr<sg4::ItemPlan>> items;  // per nvec
+  double flops_per_vec = 0, bytes_fixed = 0;
+  // optional kernel timing: HIP events around the H.psi launch of each apply
+  bool timing = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pairs;
+  size_t ev_used = 0;
+  ~sg4_plan() {
+    for (auto& e : ev_pairs) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+  }
+};
+
+namespace sg4 {
+
+static int build_terms(sg4_plan* P) {
+  const int D = P->D, L1 = P->Lmax + 1;
+  P->h_terms.resize(P->nterms);
+  std::vector<double> ck = P->h_ck;
+  if (ck.empty()) ck.assign(D, 0.0);
+  for (int64_t t = 0; t < P->nterms; ++t) {
+    TermDev T{};
+    T.boff = P->h_boff[t];
+    T.qoff = P->h_qoff[t];
+    T.nb = (int32_t)(P->h_boff[t + 1] - P->h_boff[t]);
+    T.nq = (int32_t)(P->h_qoff[t + 1] - P->h_qoff[t]);
+    double w = P->h_weights[t], vs = 0.0;
+    int nact = 0;
+    for (int k = 0; k < D; ++k) {
+      const int l = P->h_levels[(size_t)t * D + k];
+      const int idx = k * L1 + l;
+      const MatDev& m = P->h_md[idx];
+      if (m.nq == 1 && m.nb == 1 && P->type_op == 1) {
+        w *= m.b00 * m.btw00;
+        vs += ck[k] * m.d200;
+      } else {
+        if (nact >= MAXACT) {
+          set_error("more than 16 modes with more than one basis function in a term");
+          return SG4_ERR_UNSUPPORTED;
+        }
+        T.mat[nact] = (int16_t)idx;
+        T.kmode[nact] = (int8_t)k;
+        ++nact;
+      }
+    }
+    T.nact = nact;
+    T.wgt = w;
+    T.vshift = vs;
+    P->h_terms[t] = T;
+  }
+  int rc = P->d_terms.alloc(sizeof(TermDev) * P->h_terms.size());
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(P->d_terms.p, P->h_terms.data(), sizeof(TermDev) * P->h_terms.size(),
+                    hipMemcpyHostToDevice));
+  return SG4_OK;
+}
+
+// Work items for a block of nvec vectors: for every term the vectors are cut
+// into power-of-two chunks so that both work buffers fit the LDS budget; a
+// term too large for that even with one vector keeps its second buffer in a
+// global scratch slot.  Items are ordered by decreasing work so that the
+// dispatcher starts the long ones first.
+static int get_items(sg4_plan* P, int nvec, ItemPlan** out) {
+  auto itp = P->items.find(nvec);
+  if (itp != P->items.end()) { *out = itp->second.get(); return SG4_OK; }
+  struct HI { Item it; double cost; };
+  std::vector<HI> all;
+  all.reserve(P->nterms * 2);
+  int64_t nbig = 0, slot = 0;
+  int lds = 0;
+  for (int64_t t = 0; t < P->nterms; ++t) {
+    const TermDev& T = P->h_terms[t];
+    const int64_t nmax = std::max(T.nq, T.nb);
+    const int64_t per = 2 * nmax * 8;  // bytes per vector (two buffers)
+    const bool big = per > LDS_SMALL;
+    if (big && nmax * 8 > LDS_SMALL) {
+      set_error("term with " + std::to_string(nmax) + " grid points exceeds the LDS capacity");
+      return SG4_ERR_UNSUPPORTED;
+    }
+    int gmax = 0;
+    if (!big)
+      while ((1 << (gmax + 1)) <= nvec && per * (1 << (gmax + 1)) <= LDS_SMALL) ++gmax;
+    double work = 0;
+    for (int p = 0; p < T.nact; ++p) work += P->h_md[T.mat[p]].nq;
+    work = (work + 1.0) * (double)nmax;
+    int v0 = 0;
+    while (v0 < nvec) {
+      int g = gmax;
+      while ((1 << g) > nvec - v0) --g;
+      HI h{};
+      h.it.term = (int32_t)t;
+      h.it.v0 = (int16_t)v0;
+      h.it.lgnv = (int8_t)g;
+      h.it.big = big ? 1 : 0;
+      h.it.slot = big ? (int32_t)(nbig++) : 0;
+      lds = std::max<int>(lds, (int)(big ? nmax * 8 : per << g));
+      slot = std::max<int64_t>(slot, big ? nmax : 0);
+      h.cost = work * (1 << g);
+      all.push_back(h);
+      v0 += 1 << g;
+    }
+  }
+  std::stable_sort(all.begin(), all.end(), [](const HI& x, const HI& y) { return x.cost > y.cost; });
+  auto ip = std::make_unique<ItemPlan>();
+  std::vector<Item> h(all.size());
+  for (size_t i = 0; i < all.size(); ++i) h[i] = all[i].it;
+  ip->nitems = (int64_t)h.size();
+  ip->nbig = nbig;
+  ip->slot_size = slot;
+  ip->lds = lds;
+  int rc = ip->items.alloc(sizeof(Item) * h.size());
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(ip->items.p, h.data(), sizeof(Item) * h.size(), hipMemcpyHostToDevice));
+  if (nbig > 0 && (rc = ip->scratch.alloc(sizeof(double) * nbig * slot))) return rc;
+  *out = ip.get();
+  P->items[nvec] = std::move(ip);
+  return SG4_OK;
+}
+
+static int grid_blocks(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  return (int)std::min<int64_t>(std::max<int64_t>(b, 1), 8192);
+}
+
+static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, int prec) {
+  if (P->type_op == 0) { set_error("no operator set on the plan"); return SG4_ERR_STATE; }
+  if (nvec < 1 || nvec > P->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
+  if (prec != SG4_PREC_FP64) { set_error("precision mode not available"); return SG4_ERR_UNSUPPORTED; }
+  if (P->type_op != 1) { set_error("operator type not available"); return SG4_ERR_UNSUPPORTED; }
+  ItemPlan* ip = nullptr;
+  int rc = get_items(P, nvec, &ip);
+  if (rc) return rc;
+  const int64_t n = P->nb_ba * nvec;
+  hipLaunchKernelGGL(k_fill, dim3(grid_blocks(n)), dim3(256), 0, P->stream, hpsi, n, 0.0);
+  KArgs a;
+  a.terms = P->d_terms.as<TermDev>();
+  a.md = P->d_md.as<MatDev>();
+  a.mats = P->d_mats.as<double>();
+  a.map = P->d_map.as<int32_t>();
+  a.V = P->d_V.as<double>();
+  a.ck = P->d_ck.as<double>();
+  a.psi = psi;
+  a.hpsi = hpsi;
+  a.nvec = nvec;
+  a.items = ip->items.as<Item>();
+  a.scratch = ip->scratch.as<double>();
+  a.slot_size = ip->slot_size;
+  if (P->timing) {
+    if (P->ev_used == P->ev_pairs.size()) {
+      hipEvent_t e0, e1;
+      HIP_TRY(hipEventCreate(&e0));
+      HIP_TRY(hipEventCreate(&e1));
+      P->ev_pairs.emplace_back(e0, e1);
+    }
+    P->ev0 = P->ev_pairs[P->ev_used].first;
+    P->ev1 = P->ev_pairs[P->ev_used].second;
+    ++P->ev_used;
+    HIP_TRY(hipEventRecord(P->ev0, P->stream));
+  }
+  hipLaunchKernelGGL(k_hpsi_t1_f64, dim3((unsigned)ip->nitems), dim3(NT), ip->lds, P->stream, a);
+  if (P->timing) HIP_TRY(hipEventRecord(P->ev1, P->stream));
+  HIP_TRY(hipGetLastError());
+  return SG4_OK;
+}
+
+}  // namespace sg4
+
+using namespace sg4;
+
+extern "C" {
+
+const char* sg4_last_error(void) { return g_last_error.c_str(); }
+
+int sg4_setup_create(int32_t D, int32_t L, int32_t LB, const sg4_mode_desc* modes,
+                     sg4_setup** out) {
+  if (!modes || !out) { set_error("null argument"); return SG4_ERR_ARG; }
+  try {
+    auto* s = new sg4_setup;
+    s->s = new Setup(D, L, LB, modes);
+    *out = s;
+    return SG4_OK;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return SG4_ERR_ARG;
+  }
+}
+
+int sg4_setup_sizes(const sg4_setup* s, int64_t* nterms, int64_t* nb_ba, int64_t* sum_nb,
+                    int64_t* sum_nq) {
+  if (!s) { set_error("null setup"); return SG4_ERR_ARG; }
+  if (nterms) *nterms = s->s->nterms;
+  if (nb_ba) *nb_ba = s->s->nb_ba;
+  if (sum_nb) *sum_nb = s->s->boff.back();
+  if (sum_nq) *sum_nq = s->s->qoff.back();
+  return SG4_OK;
+}
+
+int sg4_setup_tables(const sg4_setup* s, sg4_tables* out) {
+  if (!s || !out) { set_error("null argument"); return SG4_ERR_ARG; }
+  s->s->fill_tables(out);
+  return SG4_OK;
+}
+
+int sg4_setup_term_grid(const sg4_setup* s, int64_t t, double* q) {
+  if (!s || !q || t < 0 || t >= s->s->nterms) { set_error("bad term"); return SG4_ERR_ARG; }
+  s->s->term_grid(t, q);
+  return SG4_OK;
+}
+
+int sg4_setup_basis1d(const sg4_setup* s, int32_t k, int32_t l, int32_t* nq, int32_t* nb,
+                      double* x, double* w, double* d0, double* d1, double* d2) {
+  if (!s || k < 0 || k >= s->s->D || l < 0 || l > s->s->L) { set_error("bad mode/level"); return SG4_ERR_ARG; }
+  const Basis1D& b = s->s->b1[k][l];
+  if (nq) *nq = b.nq;
+  if (nb) *nb = b.nb;
+  if (x) std::memcpy(x, b.x.data(), sizeof(double) * b.nq);
+  if (w) std::memcpy(w, b.w.data(), sizeof(double) * b.nq);
+  size_t qb = (size_t)b.nq * b.nb;
+  if (d0) std::memcpy(d0, b.d0.data(), sizeof(double) * qb);
+  if (d1) std::memcpy(d1, b.d1.data(), sizeof(double) * qb);
+  if (d2) std::memcpy(d2, b.d2.data(), sizeof(double) * qb);
+  return SG4_OK;
+}
+
+int sg4_setup_destroy(sg4_setup* s) {
+  if (s) { delete s->s; delete s; }
+  return SG4_OK;
+}
+
+int sg4_plan_create(const sg4_tables* t, int32_t max_nvec, sg4_plan** out) {
+  if (!t || !out || !t->levels || !t->weights || !t->map || !t->nq || !t->nb || !t->mats) {
+    set_error("null table pointer");
+    return SG4_ERR_ARG;
+  }
+  if (t->D < 1 || t->D > SG4_MAX_D || t->Lmax < 0 || t->Lmax > SG4_MAX_L || t->nterms < 1 ||
+      t->nb_ba < 1 || t->nb_ba > INT32_MAX || max_nvec < 1 || max_nvec > 4096) {
+    set_error("table sizes out of range");
+    return SG4_ERR_ARG;
+  }
+  static std::once_flag attr_once;
+  std::call_once(attr_once, [] {
+    (void)hipFuncSetAttribute((const void*)k_hpsi_t1_f64,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
+  });
+  std::unique_ptr<sg4_plan> P(new sg4_plan);
+  P->D = t->D; P->Lmax = t->Lmax; P->nterms = t->nterms; P->nb_ba = t->nb_ba;
+  P->max_nvec = max_nvec;
+  const int D = t->D, L1 = t->Lmax + 1;
+  P->h_nq.assign(t->nq, t->nq + (size_t)D * L1);
+  P->h_nb.assign(t->nb, t->nb + (size_t)D * L1);
+  // matrices
+  int64_t off = 0, noff = 0;
+  P->h_md.resize((size_t)D * L1);
+  std::vector<int64_t> node_off((size_t)D * L1);
+  for (int k = 0; k < D; ++k)
+    for (int l = 0; l < L1; ++l) {
+      const int idx = k * L1 + l;
+      const int nq = P->h_nq[idx], nb = P->h_nb[idx];
+      if (nq < 1 || nb < 1 || nb > nq || nq > SG4_MAX_N) {
+        set_error("1-D sizes out of range (need 1 <= nb <= nq <= 64)");
+        return SG4_ERR_ARG;
+      }
+      MatDev& m = P->h_md[idx];
+      m.nq = nq; m.nb = nb;
+      m.B = off; off += (int64_t)nq * nb;
+      m.BtW = off; off += (int64_t)nq * nb;
+      m.D1 = off; off += (int64_t)nq * nq;
+      m.D2 = off; off += (int64_t)nq * nq;
+      m.b00 = t->mats[m.B]; m.btw00 = t->mats[m.BtW]; m.d200 = t->mats[m.D2];
+      node_off[idx] = noff; noff += nq;
+    }
+  // terms
+  P->h_levels.assign(t->levels, t->levels + (size_t)D * t->nterms);
+  P->h_weights.assign(t->weights, t->weights + t->nterms);
+  P->h_boff.resize(t->nterms + 1);
+  P->h_qoff.resize(t->nterms + 1);
+  P->h_boff[0] = P->h_qoff[0] = 0;
+  for (int64_t tt = 0; tt < t->nterms; ++tt) {
+    int64_t nbt = 1, nqt = 1;
+    for (int k = 0; k < D; ++k) {
+      const int l = P->h_levels[(size_t)tt * D + k];
+      if (l < 0 || l > t->Lmax) { set_error("term level out of range"); return SG4_ERR_ARG; }
+      nbt *= P->h_nb[k * L1 + l];
+      nqt *= P->h_nq[k * L1 + l];
+    }
+    if (nqt > INT32_MAX / 64) { set_error("term too large"); return SG4_ERR_UNSUPPORTED; }
+    P->h_boff[tt + 1] = P->h_boff[tt] + nbt;
+    P->h_qoff[tt + 1] = P->h_qoff[tt] + nqt;
+  }
+  P->sum_nb = P->h_boff.back();
+  P->sum_nq = P->h_qoff.back();
+  for (int64_t e = 0; e < P->sum_nb; ++e)
+    if (t->map[e] < -1 || t->map[e] >= t->nb_ba) { set_error("map entry out of range"); return SG4_ERR_ARG; }
+  int rc;
+  if ((rc = P->d_mats.alloc(sizeof(double) * off))) return rc;
+  HIP_TRY(hipMemcpy(P->d_mats.p, t->mats, sizeof(double) * off, hipMemcpyHostToDevice));
+  if ((rc = P->d_md.alloc(sizeof(MatDev) * P->h_md.size()))) return rc;
+  HIP_TRY(hipMemcpy(P->d_md.p, P->h_md.data(), sizeof(MatDev) * P->h_md.size(), hipMemcpyHostToDevice));
+  if ((rc = P->d_map.alloc(sizeof(int32_t) * P->sum_nb))) return rc;
+  HIP_TRY(hipMemcpy(P->d_map.p, t->map, sizeof(int32_t) * P->sum_nb, hipMemcpyHostToDevice));
+  if ((rc = P->d_levels.alloc(sizeof(int32_t) * P->h_levels.size()))) return rc;
+  HIP_TRY(hipMemcpy(P->d_levels.p, P->h_levels.data(), sizeof(int32_t) * P->h_levels.size(), hipMemcpyHostToDevice));
+  if ((rc = P->d_qoff.alloc(sizeof(int64_t) * P->h_qoff.size()))) return rc;
+  HIP_TRY(hipMemcpy(P->d_qoff.p, P->h_qoff.data(), sizeof(int64_t) * P->h_qoff.size(), hipMemcpyHostToDevice));
+  if ((rc = P->d_nqtab.alloc(sizeof(int32_t) * P->h_nq.size()))) return rc;
+  HIP_TRY(hipMemcpy(P->d_nqtab.p, P->h_nq.data(), sizeof(int32_t) * P->h_nq.size(), hipMemcpyHostToDevice));
+  if (t->nodes) {
+    if ((rc = P->d_nodes.alloc(sizeof(double) * noff))) return rc;
+    HIP_TRY(hipMemcpy(P->d_nodes.p, t->nodes, sizeof(double) * noff, hipMemcpyHostToDevice));
+    if ((rc = P->d_node_off.alloc(sizeof(int64_t) * node_off.size()))) return rc;
+    HIP_TRY(hipMemcpy(P->d_node_off.p, node_off.data(), sizeof(int64_t) * node_off.size(), hipMemcpyHostToDevice));
+    P->have_nodes = true;
+  }
+  if ((rc = P->d_V.alloc(sizeof(double) * P->sum_nq))) return rc;
+  // work buffers for the column-major entry point
+  const size_t blk = sizeof(double) * (size_t)P->nb_ba * max_nvec;
+  if ((rc = P->d_rows_in.alloc(blk))) return rc;
+  if ((rc = P->d_rows_out.alloc(blk))) return rc;
+  // algorithmic flop / byte counts (type 1): per vector 3 * sum_t S_t,
+  // S_t = sum_k 2 N_t n_k; bytes: V grid + map (+ 16 nb_ba per vector)
+  double S = 0;
+  for (int64_t tt = 0; tt < t->nterms; ++tt) {
+    const double Nt = (double)(P->h_qoff[tt + 1] - P->h_qoff[tt]);
+    for (int k = 0; k < D; ++k) S += 2.0 * Nt * P->h_nq[k * L1 + P->h_levels[(size_t)tt * D + k]];
+  }
+  P->flops_per_vec = 3.0 * S;
+  P->bytes_fixed = 8.0 * P->sum_nq + 4.0 * P->sum_nb;
+  *out = P.release();
+  return SG4_OK;
+}
+
+int sg4_plan_set_stream(sg4_plan* p, void* s) {
+  if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
+  p->stream = (hipStream_t)s;
+  return SG4_OK;
+}
+
+int sg4_plan_set_operator(sg4_plan* p, const sg4_operator* op) {
+  if (!p || !op || !op->V) { set_error("null operator"); return SG4_ERR_ARG; }
+  if (op->type_op != 1) { set_error("only type_op = 1 is available"); return SG4_ERR_UNSUPPORTED; }
+  if (!op->ck) { set_error("type 1 needs ck"); return SG4_ERR_ARG; }
+  p->type_op = 1;
+  p->h_ck.assign(op->ck, op->ck + p->D);
+  int rc;
+  if ((rc = p->d_ck.alloc(sizeof(double) * p->D))) return rc;
+  HIP_TRY(hipMemcpy(p->d_ck.p, op->ck, sizeof(double) * p->D, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(p->d_V.p, op->V, sizeof(double) * p->sum_nq, hipMemcpyHostToDevice));
+  p->items.clear();
+  return build_terms(p);
+}
+
+int sg4_plan_set_henon_heiles(sg4_plan* p, double lambda) {
+  if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
+  if (!p->have_nodes) { set_error("plan was created without grid nodes"); return SG4_ERR_STATE; }
+  HHArgs a;
+  a.levels = p->d_levels.as<int32_t>();
+  a.qoff = p->d_qoff.as<int64_t>();
+  a.nodes = p->d_nodes.as<double>();
+  a.node_off = p->d_node_off.as<int64_t>();
+  a.nqtab = p->d_nqtab.as<int32_t>();
+  a.D = p->D; a.Lmax = p->Lmax; a.nterms = p->nterms; a.lambda = lambda;
+  a.V = p->d_V.as<double>();
+  hipLaunchKernelGGL(k_grid_hh, dim3((unsigned)std::min<int64_t>(p->nterms, 65535)), dim3(256), 0,
+                     p->stream, a);
+  HIP_TRY(hipGetLastError());
+  p->type_op = 1;
+  p->h_ck.assign(p->D, -0.5);
+  int rc;
+  if ((rc = p->d_ck.alloc(sizeof(double) * p->D))) return rc;
+  HIP_TRY(hipMemcpy(p->d_ck.p, p->h_ck.data(), sizeof(double) * p->D, hipMemcpyHostToDevice));
+  p->items.clear();
+  rc = build_terms(p);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return SG4_OK;
+}
+
+int sg4_apply_rows(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_t prec) {
+  if (!p || !psi || !hpsi) { set_error("null argument"); return SG4_ERR_ARG; }
+  return apply_rows(p, nvec, psi, hpsi, prec);
+}
+
+int sg4_apply(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_t prec) {
+  if (!p || !psi || !hpsi) { set_error("null argument"); return SG4_ERR_ARG; }
+  if (nvec < 1 || nvec > p->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
+  const int64_t n = p->nb_ba * nvec;
+  double* ri = p->d_rows_in.as<double>();
+  double* ro = p->d_rows_out.as<double>();
+  if (nvec == 1) {
+    return apply_rows(p, 1, psi, hpsi, prec);
+  }
+  hipLaunchKernelGGL(k_cols_to_rows, dim3(grid_blocks(n)), dim3(256), 0, p->stream, psi, ri, p->nb_ba, nvec);
+  int rc = apply_rows(p, nvec, ri, ro, prec);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_rows_to_cols, dim3(grid_blocks(n)), dim3(256), 0, p->stream, ro, hpsi, p->nb_ba, nvec);
+  HIP_TRY(hipGetLastError());
+  return SG4_OK;
+}
+
+int sg4_apply_host(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_t prec) {
+  if (!p || !psi || !hpsi) { set_error("null argument"); return SG4_ERR_ARG; }
+  if (nvec < 1 || nvec > p->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
+  const size_t bytes = sizeof(double) * (size_t)p->nb_ba * nvec;
+  int rc;
+  if (p->d_cols_in.n < bytes) {
+    if ((rc = p->d_cols_in.alloc(sizeof(double) * (size_t)p->nb_ba * p->max_nvec))) return rc;
+    if ((rc = p->d_cols_out.alloc(sizeof(double) * (size_t)p->nb_ba * p->max_nvec))) return rc;
+  }
+  HIP_TRY(hipMemcpyAsync(p->d_cols_in.p, psi, bytes, hipMemcpyHostToDevice, p->stream));
+  rc = sg4_apply(p, nvec, p->d_cols_in.as<double>(), p->d_cols_out.as<double>(), prec);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(hpsi, p->d_cols_out.p, bytes, hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return SG4_OK;
+}
+
+int sg4_plan_stats(const sg4_plan* p, int32_t nvec, double* flops, double* bytes, int64_t* items) {
+  if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
+  if (flops) *flops = p->flops_per_vec * nvec;
+  if (bytes) *bytes = p->bytes_fixed + 16.0 * (double)p->nb_ba * nvec;
+  if (items) {
+    auto it = p->items.find(nvec);
+    *items = (it == p->items.end()) ? -1 : it->second->nitems;
+  }
+  return SG4_OK;
+}
+
+int sg4_plan_timing(sg4_plan* p, int32_t enable) {
+  if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
+  p->timing = enable != 0;
+  p->ev_used = 0;
+  return SG4_OK;
+}
+
+int sg4_plan_kernel_times(sg4_plan* p, float* ms, int32_t max, int32_t* n) {
+  if (!p || !n) { set_error("null argument"); return SG4_ERR_ARG; }
+  int32_t k = 0;
+  for (size_t i = 0; i < p->ev_used && k < max; ++i, ++k) {
+    HIP_TRY(hipEventSynchronize(p->ev_pairs[i].second));
+    HIP_TRY(hipEventElapsedTime(&ms[k], p->ev_pairs[i].first, p->ev_pairs[i].second));
+  }
+  *n = k;
+  p->ev_used = 0;
+  return SG4_OK;
+}
+
+int sg4_plan_destroy(sg4_plan* p) {
+  delete p;
+  return SG4_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,161 @@
+/*
+ * sg4hpsi.h -- C ABI of the MI355X (gfx950) Smolyak SG4 Hamiltonian action.
+ *
+ * Drop-in boundary for ElVibRot's SG type 4 operator action.  The Fortran
+ * side keeps its OpPsi / Davidson / propagation drivers and binds these entry
+ * points through iso_c_binding (module mod_sg4_gpu in
+ * elvibrot_amd/fortran/mod_sg4_gpu.f90).  All arguments are plain pointers
+ * and sizes; no C++ or torch types cross this boundary.
+ *
+ * Conventions
+ *   - integers are int32 unless noted; sizes that can exceed 2^31 are int64;
+ *   - arrays are column-major, first index fastest (Fortran order);
+ *   - packed vectors are nb_ba x nvec blocks (Psi(v)%RvecB(:) of a block of
+ *     param_psi, SRC/sub_WP/sub_module_psi_set_alloc.f90:56-169);
+ *   - every call returns SG4_OK (0) or a negative status; sg4_last_error()
+ *     gives the message.  The Fortran wrapper maps a nonzero status to STOP,
+ *     as the reference does (sub_OpPsi_SG4.f90:741-753).
+ *   - one host thread per plan; kernels run on the plan's HIP stream.
+ */
+#ifndef SG4HPSI_H
+#define SG4HPSI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SG4_OK 0
+#define SG4_ERR_ARG (-1)         /* bad argument or size */
+#define SG4_ERR_HIP (-2)         /* HIP runtime error */
+#define SG4_ERR_OOM (-3)         /* device allocation failed */
+#define SG4_ERR_UNSUPPORTED (-4) /* valid input outside what the kernels handle */
+#define SG4_ERR_STATE (-5)       /* call out of order (e.g. no operator set) */
+
+#define SG4_BASIS_HO 0       /* 1-D harmonic oscillator ("Hm"), Gauss-Hermite grid */
+#define SG4_BASIS_LEGENDRE 1 /* 1-D Legendre in theta ("Pl0"), Gauss-Legendre grid */
+
+/* precision modes of sg4_apply */
+#define SG4_PREC_FP64 0      /* FP64 throughout (parity mode, 1e-12 relative) */
+#define SG4_PREC_F32 1       /* mode contractions in FP32, FP64 gather/scatter */
+#define SG4_PREC_F32X2 2     /* split FP32 (hi+lo) contractions on MFMA, FP32 accumulation */
+
+typedef struct sg4_setup sg4_setup; /* host tables built by this library */
+typedef struct sg4_plan sg4_plan;   /* device state of one SG4 operator */
+
+/* ---- 1-D basis description (one per mode), as in &basis_nD -------------- */
+typedef struct {
+  int32_t kind;          /* SG4_BASIS_HO or SG4_BASIS_LEGENDRE */
+  int32_t nq_A, nq_B;    /* nq(l) = nq_A + nq_B*l   (L_TO_nq_A/B) */
+  int32_t nb_A, nb_B;    /* nb(l) = nb_A + nb_B*l   (defaults: = nq rule) */
+  double Q0, scaleQ;     /* Q = Q0 + x/scaleQ (HO only) */
+} sg4_mode_desc;
+
+/* ---- tables consumed by the device plan --------------------------------
+ * Field-by-field counterpart of what BasisnD holds after
+ * RecSparseGrid_ForDP_type4 (SRC/sub_Basis/sub_quadra_SparseBasis.f90:922):
+ *   levels  = para_SGType2%nDind_SmolyakRep%Tab_nDval   (D x nterms)
+ *   weights = WeightSG                                   (nterms)
+ *   map     = para_SGType2%tab_iB_OF_SRep_TO_iB - 1      (sum_t nb_t; -1 = absent)
+ *   nq, nb  = sizes of tab_basisPrimSG(l,k)              ((Lmax+1) x D)
+ *   mats    = for k = 1..D, for l = 0..Lmax, concatenated:
+ *               dnRGB%d0 (nq x nb), dnRBGwrho%d0 (nb x nq),
+ *               dnRGG%d1 (nq x nq), dnRGG%d2 (nq x nq)
+ * Term t occupies map[boff(t) : boff(t)+nb_t) with boff the running sum of
+ * nb_t = prod_k nb(levels(k,t),k); its grid occupies [qoff(t), qoff(t)+nq_t)
+ * of every grid array, with the in-term index first-index-fastest.        */
+typedef struct {
+  int32_t D;
+  int32_t Lmax;
+  int64_t nterms;
+  int64_t nb_ba;
+  const int32_t *levels;
+  const double *weights;
+  const int32_t *map;
+  const int32_t *nq;
+  const int32_t *nb;
+  const double *mats;
+  const double *nodes;   /* optional: for k, for l: the nq grid nodes (basis%x) */
+} sg4_tables;
+
+/* ---- operator on the grids (get_OpGrid_type{1,10}_OF_ONEDP_FOR_SG4,
+ * SRC/sub_Operator/sub_OpPsi_SG4.f90:2536-3082) ------------------------------
+ * type_op = 1  : H = V(q) + sum_k ck[k] d^2/dq_k^2  (Type_HamilOp=1, constant
+ *                diagonal G: ck = -G_kk/2)
+ * type_op = 10 : H = -1/2 (1/(Jac sqRJ)) sum_ij d_i Jac G_ij d_j sqRJ + V
+ * Grid arrays have sum_t nq_t points; G is D x D per point (G[i + D*j + D*D*p]). */
+typedef struct {
+  int32_t type_op;
+  const double *V;       /* required */
+  const double *ck;      /* type 1: D coefficients */
+  const double *G;       /* type 10 */
+  const double *Jac;     /* type 10 */
+  const double *sqRJ;    /* type 10: sqRhoOVERJac */
+} sg4_operator;
+
+const char *sg4_last_error(void);
+
+/* ---- set-up (replaces RecSparseGrid_ForDP_type4 + sub_dnGB_TO_dnGG when the
+ * host does not build the tables itself; SRC/sub_Basis/
+ * sub_quadra_SparseBasis.f90:922-1434, SRC/sub_Basis/sub_module_basis.f90:2048) */
+int sg4_setup_create(int32_t D, int32_t L, int32_t LB, const sg4_mode_desc *modes,
+                     sg4_setup **out);
+/* nterms, nb_ba, sum_t nb_t, sum_t nq_t */
+int sg4_setup_sizes(const sg4_setup *s, int64_t *nterms, int64_t *nb_ba, int64_t *sum_nb,
+                    int64_t *sum_nq);
+/* pointers into setup-owned memory, valid until sg4_setup_destroy */
+int sg4_setup_tables(const sg4_setup *s, sg4_tables *out);
+/* grid coordinates of term t: q[k + D*p], p < nq_t (Rec_Qact_SG4_with_Tab_iq,
+ * SRC/sub_Basis/sub_module_basis.f90:3979) */
+int sg4_setup_term_grid(const sg4_setup *s, int64_t t, double *q);
+/* 1-D rule and basis of (mode k, level l): x, w (nq), d0, d1, d2 (nq x nb) */
+int sg4_setup_basis1d(const sg4_setup *s, int32_t k, int32_t l, int32_t *nq, int32_t *nb,
+                      double *x, double *w, double *d0, double *d1, double *d2);
+int sg4_setup_destroy(sg4_setup *s);
+
+/* ---- device plan ----------------------------------------------------------
+ * Owns device copies of the tables, the 1-D matrices and the grids, plus
+ * pre-sized work buffers for blocks of up to max_nvec vectors: no device
+ * allocation happens inside sg4_apply.                                       */
+int sg4_plan_create(const sg4_tables *tables, int32_t max_nvec, sg4_plan **out);
+/* hip_stream: a hipStream_t (NULL = default stream) */
+int sg4_plan_set_stream(sg4_plan *p, void *hip_stream);
+/* copy the operator grids to the device (host pointers) */
+int sg4_plan_set_operator(sg4_plan *p, const sg4_operator *op);
+/* evaluate the Henon-Heiles grid V on the device (first-call grid evaluation
+ * of get_OpGrid_type1_OF_ONEDP_FOR_SG4, sub_OpPsi_SG4.f90:2978-3058, with
+ * calcN_op of Working_tests/MPI_tests/6D_Davidson_S0/sub_system_HenonHeiles.f:14-43),
+ * kinetic -1/2 sum d^2 (type 1).  Needs tables->nodes at plan creation. */
+int sg4_plan_set_henon_heiles(sg4_plan *p, double lambda);
+
+/* Block H.psi: sub_TabOpPsi_FOR_SGtype4(Psi, OpPsi, para_Op)
+ * (SRC/sub_Operator/sub_OpPsi_SG4.f90:681-702).  psi and hpsi are DEVICE
+ * pointers to nb_ba x nvec column-major blocks; hpsi is overwritten. */
+int sg4_apply(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);
+/* Same with HOST pointers (copies in and out over PCIe; synchronous). */
+int sg4_apply_host(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);
+/* Same in the interleaved device layout used by the device-side drivers:
+ * psi[row*nvec + v], row < nb_ba. */
+int sg4_apply_rows(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);
+
+/* Statistics of the last apply: launches, items, and the algorithmic flop and
+ * byte counts per block apply used for roofline reporting. */
+int sg4_plan_stats(const sg4_plan *p, int32_t nvec, double *flops, double *bytes,
+                   int64_t *items);
+/* Kernel timing for roofline reporting: when enabled, every apply records a
+ * pair of HIP events on the plan's stream around its H.psi kernel launch;
+ * sg4_plan_kernel_times waits for them, returns the elapsed milliseconds of
+ * up to max recorded launches in order, and clears the record. */
+int sg4_plan_timing(sg4_plan *p, int32_t enable);
+int sg4_plan_kernel_times(sg4_plan *p, float *ms, int32_t max, int32_t *n);
+int sg4_plan_destroy(sg4_plan *p);
+
+#define SG4_MAX_D 32
+#define SG4_MAX_L 40
+#define SG4_MAX_N 64
+
+#ifdef __cplusplus
+}
+#endif
+#endif

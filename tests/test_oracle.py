@@ -1,0 +1,192 @@
+"""The oracle pinned against the reference's own fixtures and against the
+mathematics (CPU only).
+
+Fixtures (tests/golden/, copied by fetch_reference_fixtures.sh):
+  hermquadra/herm{n}.txt  -- Internal_data/HermQuadra (x, w, w e^{x^2}, 50 digits)
+  hh6d_L3_benchmark.txt   -- Working_tests/MPI_tests/6D_Davidson_S0/benchmark
+  hh21d_L2_benchmark.txt  -- Working_tests/MPI_tests/21D_Davidson_S0/benchmark
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def read_herm(golden, n):
+    rows = [l.split() for l in open(os.path.join(golden, "hermquadra", f"herm{n}.txt")) if l.strip()]
+    x = np.array([float(r[1]) for r in rows])
+    w4 = np.array([float(r[3]) for r in rows])
+    return x, w4
+
+
+@pytest.mark.parametrize("n", list(range(1, 41)))
+def test_gauss_hermite_vs_reference_tables(golden, n):
+    # sub_quadra_herm.f90:160-178 overwrites the nodes/weights with these files
+    x, w = O.gauss_hermite(n)
+    xr, wr = read_herm(golden, n)
+    assert np.max(np.abs(x - xr)) <= 4e-15 * max(1.0, np.max(np.abs(xr)))
+    assert np.max(np.abs(w - wr) / wr) <= 1e-14
+
+
+def test_gauss_legendre_exactness():
+    for n in (1, 2, 5, 11, 21):
+        x, w = O.gauss_legendre(n)
+        for deg in range(2 * n):
+            exact = 0.0 if deg % 2 else 2.0 / (deg + 1)
+            assert abs(np.sum(w * x ** deg) - exact) < 1e-14
+
+
+@pytest.mark.parametrize("nq", [1, 3, 9, 17, 25])
+def test_ho_basis_orthonormal_and_ode(nq):
+    x, w, d0, d1, d2 = O.ho_basis(nq, nq)
+    S = d0.T @ (w[:, None] * d0)
+    assert np.max(np.abs(S - np.eye(nq))) < 1e-13
+    # Hermite ODE: phi_n'' = (x^2 - 2n - 1) phi_n
+    n = np.arange(nq)
+    assert np.max(np.abs(d2 - (x[:, None] ** 2 - 2 * n - 1) * d0)) < 1e-12
+    # ladder identity phi_n' = sqrt(n/2) phi_{n-1} - sqrt((n+1)/2) phi_{n+1}
+    for m in range(nq - 1):
+        ref = -np.sqrt((m + 1) / 2) * d0[:, m + 1]
+        if m > 0:
+            ref = ref + np.sqrt(m / 2) * d0[:, m - 1]
+        assert np.max(np.abs(d1[:, m] - ref)) < 1e-12
+
+
+def test_ho_scaling():
+    # Q = Q0 + x/scaleQ; basis stays orthonormal on the scaled weights,
+    # derivatives scale as sqrt(s) s^m (sub_scale_basis, sub_module_basis.f90:1499)
+    x0, w0, a0, a1, a2 = O.ho_basis(7, 7)
+    x, w, b0, b1, b2 = O.ho_basis(7, 7, Q0=0.5, scaleQ=2.0)
+    assert np.allclose(x, 0.5 + x0 / 2.0)
+    assert np.allclose(b1, a1 * 2 ** 1.5) and np.allclose(b2, a2 * 2 ** 2.5)
+    assert np.max(np.abs(b0.T @ (w[:, None] * b0) - np.eye(7))) < 1e-13
+
+
+@pytest.mark.parametrize("nq", [3, 9, 17])
+def test_grid_derivative_matrices_exact_on_basis(nq):
+    x, w, d0, d1, d2 = O.ho_basis(nq, nq)
+    g0, g1, g2 = O.grid_deriv(d0, d1, d2)
+    # dnRGG%dk applied to the grid values of a basis function gives its
+    # derivative values (the Check block of sub_dnGB_TO_dnGG)
+    assert np.max(np.abs(g0 - np.eye(nq))) < 1e-12
+    assert np.max(np.abs(g1 @ d0 - d1)) < 1e-11
+    assert np.max(np.abs(g2 @ d0 - d2)) < 1e-11
+
+
+def test_grid_derivative_rectangular():
+    x, w, d0, d1, d2 = O.ho_basis(9, 5)
+    g0, g1, g2 = O.grid_deriv(d0, d1, d2)
+    assert np.max(np.abs(g1 @ d0 - d1)) < 1e-11
+    assert np.max(np.abs(g2 @ d0 - d2)) < 1e-11
+
+
+@pytest.mark.parametrize("D,L,nterms,nb_ba,sum_nq", [
+    (6, 3, 84, 377, 1216), (6, 6, 923, 8989, 93535), (21, 2, 253, 925, 2059),
+    (3, 10, 166, None, None)])
+def test_smolyak_counts_and_weights(D, L, nterms, nb_ba, sum_nq):
+    s = O.SG4(D, L)
+    Lmin = max(0, L - D + 1)
+    # number of terms: sum_{m=Lmin}^{L} C(m+D-1, D-1)
+    n = sum(math.comb(m + D - 1, D - 1) for m in range(Lmin, L + 1))
+    assert s.nterms == n
+    if nterms is not None and nb_ba is not None:
+        assert s.nterms == nterms and s.nb_ba == nb_ba and s.sum_nq == sum_nq
+    # the combination technique reproduces constants: sum of weights is 1
+    assert abs(s.weight.sum() - 1.0) < 1e-12
+    # packed basis count from the generating function of lambda(i)=ceil((i-1)/2)
+    poly = np.zeros(L + 1, dtype=object); poly[0] = 1
+    c = [1] + [2] * L
+    for _ in range(D):
+        new = np.zeros(L + 1, dtype=object)
+        for a in range(L + 1):
+            for b in range(L + 1 - a):
+                new[a + b] += poly[a] * c[b]
+        poly = new
+    assert s.nb_ba == int(poly.sum())
+    # every packed function is reached by at least one term, map rows valid
+    assert s.map.min() >= 0 and s.map.max() == s.nb_ba - 1
+    assert len(np.unique(s.map)) == s.nb_ba
+
+
+def test_term_order_first_index_fastest():
+    s = O.SG4(3, 2)
+    # ADD_ONE_TO_nDindex_type5m order for D=3, L=2, Lmin=0
+    expect = [(0,0,0),(1,0,0),(2,0,0),(0,1,0),(1,1,0),(0,2,0),(0,0,1),(1,0,1),(0,1,1),(0,0,2)]
+    assert [tuple(r) for r in s.lev] == expect
+
+
+def test_packed_order_last_index_fastest():
+    s = O.SG4(2, 1)
+    # lambda(i)=ceil((i-1)/2) (1-based): index 0 -> 0, 1,2 -> 1 ; sum <= 1
+    expect = [(0, 0), (0, 1), (0, 2), (1, 0), (2, 0)]
+    assert [tuple(r) for r in s.packed] == expect
+
+
+def test_smolyak_exact_for_separable_polynomial_potential():
+    # For V = sum_k a_k q_k^2 (separable quadratic) the SG4 matrix of a
+    # tensor basis is exact where every 1-D factor is exactly integrated; the
+    # diagonal of the operator on the lowest function is sum_k (a_k/2 + 1/4).
+    s = O.SG4(4, 3)
+    a = np.array([0.5, 0.7, 1.1, 0.3])
+    V = np.concatenate([(s.term_grid(t) ** 2 @ a) for t in range(s.nterms)])
+    e0 = np.zeros(s.nb_ba); e0[0] = 1.0
+    h = s.hpsi(e0, V, -0.5 * np.ones(4))
+    assert abs(h[0] - np.sum(a / 2 + 0.25)) < 1e-12
+
+
+def _dense_H(s, V, ck):
+    H = s.hpsi(np.eye(s.nb_ba), V, ck)
+    return 0.5 * (H + H.T), np.max(np.abs(H - H.T))
+
+
+@pytest.mark.parametrize("D,L,fname", [(6, 3, "hh6d_L3_benchmark.txt"),
+                                       (21, 2, "hh21d_L2_benchmark.txt")])
+def test_henon_heiles_eigenvalues_vs_reference_benchmark(golden, D, L, fname):
+    """The reference's Davidson levels (absolute tolerance 1e-8 in its own
+    checker, Working_tests/MPI_tests/MPI_test.sh:39-41) are upper bounds of the
+    exact eigenvalues of the same discrete operator, converged to
+    conv_resi=5e-4 (shell_run:160-165): the exact eigenvalues must lie below
+    the benchmark by at most 1e-7 and never above it by more than 1e-10."""
+    ref = np.loadtxt(os.path.join(golden, fname))[:, 0]
+    s = O.SG4(D, L)
+    V = s.grid_hh(0.111803)
+    H, asym = _dense_H(s, V, -0.5 * np.ones(D))
+    assert asym < 1e-13
+    ev = np.linalg.eigvalsh(H)[: len(ref)]
+    d = ev - ref
+    assert d.max() < 1e-10, d
+    assert d.min() > -1.5e-7, d
+    assert abs(d[0]) < 1e-7
+
+
+def test_wrong_lambda_sign_is_caught(golden):
+    # a plausible mistake (sign of the cubic coupling) moves the levels far
+    # beyond the pin above; the symmetric-in-q term alone cannot hide it
+    ref = np.loadtxt(os.path.join(golden, "hh6d_L3_benchmark.txt"))[:, 0]
+    s = O.SG4(6, 3)
+    q = [s.term_grid(t) for t in range(s.nterms)]
+    lam = 0.111803
+    Vbad = np.concatenate([0.5 * (g ** 2).sum(1) + lam * (g[:, :-1] ** 2 * g[:, 1:] + g[:, 1:] ** 3 / 3).sum(1) for g in q])
+    H, _ = _dense_H(s, Vbad, -0.5 * np.ones(6))
+    ev = np.linalg.eigvalsh(H)[:28]
+    assert np.max(np.abs(ev - ref)) > 1e-4
+
+
+def test_type10_constant_G_matches_type1_spectrum():
+    # d1 G d1 with G = identity and Jac = sqRJ = 1 is the same continuous
+    # operator as type 1; on the grid they differ by quadrature error only,
+    # so the low eigenvalues agree closely for a converged basis
+    s = O.SG4(2, 8)
+    V = s.grid_hh(0.0)
+    D = 2
+    G = np.tile(np.eye(D).ravel(), s.sum_nq)
+    one = np.ones(s.sum_nq)
+    H1 = s.hpsi(np.eye(s.nb_ba), V, -0.5 * np.ones(D))
+    H10 = s.hpsi(np.eye(s.nb_ba), V, None, type_op=10, G=G, Jac=one, sqRJ=one)
+    e1 = np.linalg.eigvalsh(0.5 * (H1 + H1.T))[:3]
+    e10 = np.linalg.eigvalsh(0.5 * (H10 + H10.T))[:3]
+    assert np.allclose(e1, [1, 2, 2], atol=1e-10)
+    assert np.allclose(e10, [1, 2, 2], atol=1e-8)
