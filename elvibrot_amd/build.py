@@ -38,5 +38,32 @@ def build(verbose=False):
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FBUILD = os.path.join(HERE, "_build")
+
+
+def build_fortran(verbose=False):
+    """iso_c_binding module + test driver with amdflang; the driver finds
+    libsg4hpsi.so through an $ORIGIN-relative rpath."""
+    os.makedirs(FBUILD, exist_ok=True)
+    fc = "/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else "amdflang"
+    mod_o = os.path.join(FBUILD, "mod_sg4_gpu.o")
+    drv_o = os.path.join(FBUILD, "test_sg4_fortran.o")
+    exe = os.path.join(FBUILD, "test_sg4_fortran")
+    cmds = [
+        [fc, "-O2", "-c", os.path.join(FORTRAN, "mod_sg4_gpu.f90"), "-J", FBUILD, "-o", mod_o],
+        [fc, "-O2", "-c", os.path.join(FORTRAN, "test_sg4_fortran.f90"), "-J", FBUILD, "-o", drv_o],
+        [fc, drv_o, mod_o, "-L" + HERE, "-lsg4hpsi", "-Wl,-rpath,$ORIGIN/..", "-o", exe],
+    ]
+    for c in cmds:
+        if verbose:
+            print(" ".join(c), flush=True)
+        r = subprocess.run(c, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("amdflang failed")
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -77,10 +77,7 @@ struct KArgs {
   const Item* items;
   const TermDev* terms;
   const MatDev* md;
-  const double* mats;
   const int32_t* map;
-  const double* V;
-  const double* ck;
   const double* psi;
   double* hpsi;
   double* scratch;
@@ -102,16 +99,30 @@ __device__ __forceinline__ int fdiv(int n, int d, float inv) {
 enum { IN_LDS = 0, IN_GATHER = 1 };
 enum { OUT_STORE = 0, OUT_ADD = 1, OUT_VSTORE = 2, OUT_SCATTER = 3 };
 
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;   // scalar (constant) loads
+
+// the value is the same in every lane (it derives from the work item);
+// readfirstlane states that so the pointer can live in scalar registers
+template <class T>
+__device__ __forceinline__ const T* uniform_ptr(const T* p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (const T*)(((uint64_t)hi << 32) | lo);
+}
+
+template <class T>
 struct PassCtx {
-  const double* __restrict__ M;  // NO x NI, column-major
-  const double* in;
-  double* out;
+  const T* MR;   // NO x NI, row-major: MR[i*NI + j]
+  const T* in;
+  T* out;
   int AV;        // A * nv
   float invAV;
   int A;
   int nfib;
   int lgnv;
-  double coef;
+  T coef;
   // gather / scatter
   const int32_t* __restrict__ map;  // term's map slice
   const double* __restrict__ psi;
@@ -119,94 +130,101 @@ struct PassCtx {
   int nvec, v0;
   double wgt;
   // grid potential
-  const double* __restrict__ V;  // term's grid slice
-  double vshift;
+  const T* __restrict__ V;  // term's grid slice
+  T vshift;
 };
 
-template <int NI, int NO, int KIN, int KOUT>
-__device__ __forceinline__ void mode_pass(const PassCtx& c) {
+__device__ __forceinline__ double fmat(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fmat(float a, float b, float c) { return fmaf(a, b, c); }
+
+template <class T, int KOUT>
+__device__ __forceinline__ void emit(const PassCtx<T>& c, int av, int b, int i, int NO, T y, T xi) {
+  if constexpr (KOUT == OUT_STORE) {
+    c.out[av + c.AV * (i + NO * b)] = y;
+  } else if constexpr (KOUT == OUT_ADD) {
+    T* o = c.out + av + c.AV * (i + NO * b);
+    *o = fmat(c.coef, y, *o);
+  } else if constexpr (KOUT == OUT_VSTORE) {
+    const int e = (av >> c.lgnv) + c.A * (i + NO * b);
+    c.out[av + c.AV * (i + NO * b)] = fmat(c.coef, y, (c.V[e] + c.vshift) * xi);
+  } else {
+    // last active mode: nothing after it, b == 0
+    const int e = (av >> c.lgnv) + c.A * i;
+    const int m = c.map[e];
+    if (m >= 0)
+      atomicAdd(&c.hpsi[(int64_t)m * c.nvec + c.v0 + (av & ((1 << c.lgnv) - 1))],
+                c.wgt * (double)y);
+  }
+}
+
+// One mode product over all fibres of the current tensor, one fibre per
+// thread: y(i) = sum_j M(i,j) x(j).  KIN selects where the fibre comes from
+// (LDS, or the packed block through the map), KOUT what is done with y.
+// The matrix rows are read with scalar loads (wave-uniform); the pointer is
+// made opaque in every iteration so that the rows are not hoisted into
+// vector registers across the fibre loop.
+template <class T, int NI, int NO, int KIN, int KOUT>
+__device__ __forceinline__ void mode_pass(const PassCtx<T>& c) {
   for (int f = threadIdx.x; f < c.nfib; f += NT) {
     const int b = fdiv(f, c.AV, c.invAV);
     const int av = f - b * c.AV;
-    double x[NI];
+    T x[NI];
     if constexpr (KIN == IN_GATHER) {
       // first active mode: A = 1 so av is the vector index
       const int32_t* mp = c.map + NI * b;
 #pragma unroll
       for (int j = 0; j < NI; ++j) {
         const int m = mp[j];
-        x[j] = (m >= 0) ? c.psi[(int64_t)m * c.nvec + c.v0 + av] : 0.0;
+        x[j] = (m >= 0) ? (T)c.psi[(int64_t)m * c.nvec + c.v0 + av] : (T)0;
       }
     } else {
-      const double* src = c.in + av + c.AV * NI * b;
+      const T* src = c.in + av + c.AV * NI * b;
 #pragma unroll
       for (int j = 0; j < NI; ++j) x[j] = src[c.AV * j];
     }
+    cptr<T> M = (cptr<T>)uniform_ptr(c.MR);
+    asm volatile("" : "+s"(M));
 #pragma unroll
     for (int i = 0; i < NO; ++i) {
-      double y = 0.0;
+      T y = 0;
 #pragma unroll
-      for (int j = 0; j < NI; ++j) y = fma(c.M[i + NO * j], x[j], y);
-      if constexpr (KOUT == OUT_STORE) {
-        c.out[av + c.AV * (i + NO * b)] = y;
-      } else if constexpr (KOUT == OUT_ADD) {
-        double* o = c.out + av + c.AV * (i + NO * b);
-        *o = fma(c.coef, y, *o);
-      } else if constexpr (KOUT == OUT_VSTORE) {
-        const int e = (av >> c.lgnv) + c.A * (i + NO * b);
-        c.out[av + c.AV * (i + NO * b)] = fma(c.coef, y, (c.V[e] + c.vshift) * x[i]);
-      } else {
-        // last active mode: nothing after it, b == 0
-        const int e = (av >> c.lgnv) + c.A * i;
-        const int m = c.map[e];
-        if (m >= 0)
-          atomicAdd(&c.hpsi[(int64_t)m * c.nvec + c.v0 + (av & ((1 << c.lgnv) - 1))], c.wgt * y);
-      }
+      for (int j = 0; j < NI; ++j) y = fmat(M[i * NI + j], x[j], y);
+      emit<T, KOUT>(c, av, b, i, NO, y, (KOUT == OUT_VSTORE) ? x[i < NI ? i : 0] : (T)0);
     }
   }
 }
 
-// generic sizes (rectangular or > 24): runtime loops
-template <int KIN, int KOUT>
-__device__ void mode_pass_generic(const PassCtx& c, int NI, int NO) {
+// Any sizes (rectangular, > 24, or the rare terms whose second buffer is in
+// global scratch): the fibre is re-read for every output instead of being
+// held in registers, which keeps this path's register use small.
+template <class T, int KIN, int KOUT>
+__device__ __noinline__ void mode_pass_generic(const PassCtx<T>& c, int NI, int NO) {
   for (int f = threadIdx.x; f < c.nfib; f += NT) {
     const int b = fdiv(f, c.AV, c.invAV);
     const int av = f - b * c.AV;
-    double x[SG4_MAX_N];
-    for (int j = 0; j < NI; ++j) {
-      if constexpr (KIN == IN_GATHER) {
-        const int m = c.map[NI * b + j];
-        x[j] = (m >= 0) ? c.psi[(int64_t)m * c.nvec + c.v0 + av] : 0.0;
-      } else {
-        x[j] = c.in[av + c.AV * (j + NI * b)];
-      }
-    }
     for (int i = 0; i < NO; ++i) {
-      double y = 0.0;
-      for (int j = 0; j < NI; ++j) y = fma(c.M[i + NO * j], x[j], y);
-      if constexpr (KOUT == OUT_STORE) {
-        c.out[av + c.AV * (i + NO * b)] = y;
-      } else if constexpr (KOUT == OUT_ADD) {
-        double* o = c.out + av + c.AV * (i + NO * b);
-        *o = fma(c.coef, y, *o);
-      } else if constexpr (KOUT == OUT_VSTORE) {
-        const int e = (av >> c.lgnv) + c.A * (i + NO * b);
-        c.out[av + c.AV * (i + NO * b)] = fma(c.coef, y, (c.V[e] + c.vshift) * x[i]);
-      } else {
-        const int e = (av >> c.lgnv) + c.A * i;
-        const int m = c.map[e];
-        if (m >= 0)
-          atomicAdd(&c.hpsi[(int64_t)m * c.nvec + c.v0 + (av & ((1 << c.lgnv) - 1))], c.wgt * y);
+      T y = 0, xi = 0;
+      for (int j = 0; j < NI; ++j) {
+        T xj;
+        if constexpr (KIN == IN_GATHER) {
+          const int m = c.map[NI * b + j];
+          xj = (m >= 0) ? (T)c.psi[(int64_t)m * c.nvec + c.v0 + av] : (T)0;
+        } else {
+          xj = c.in[av + c.AV * (j + NI * b)];
+        }
+        if (j == i) xi = xj;
+        y = fmat(c.MR[i * NI + j], xj, y);
       }
+      emit<T, KOUT>(c, av, b, i, NO, y, xi);
     }
   }
 }
 
-template <int KIN, int KOUT>
-__device__ __forceinline__ void dispatch_pass(const PassCtx& c, int NI, int NO) {
-  if (NI == NO) {
+template <class T, bool FAST, int KIN, int KOUT>
+__device__ __forceinline__ void dispatch_pass(const PassCtx<T>& c, int NI, int NO) {
+  if (FAST && NI == NO) {
     switch (NI) {
-#define SG4_CASE(n) case n: mode_pass<n, n, KIN, KOUT>(c); return;
+#define SG4_CASE(n) case n: mode_pass<T, n, n, KIN, KOUT>(c); return;
       SG4_CASE(2) SG4_CASE(3) SG4_CASE(4) SG4_CASE(5) SG4_CASE(6) SG4_CASE(7)
       SG4_CASE(8) SG4_CASE(9) SG4_CASE(10) SG4_CASE(11) SG4_CASE(12) SG4_CASE(13)
       SG4_CASE(14) SG4_CASE(15) SG4_CASE(16) SG4_CASE(17) SG4_CASE(18) SG4_CASE(19)
@@ -215,109 +233,132 @@ __device__ __forceinline__ void dispatch_pass(const PassCtx& c, int NI, int NO) 
       default: break;
     }
   }
-  mode_pass_generic<KIN, KOUT>(c, NI, NO);
+  mode_pass_generic<T, KIN, KOUT>(c, NI, NO);
 }
 
 // ---------------------------------------------------------------------------
-// Type-1 operator, FP64: one workgroup per work item.
+// Type-1 operator: one workgroup per work item.  T is the type of the mode
+// contractions (double: parity mode; float: the FP32 mode); gather and
+// scatter stay in FP64.
 // ---------------------------------------------------------------------------
-template <bool BIG>
-__device__ __forceinline__ void term_t1_f64(const KArgs& a, const Item& it, double* lds) {
-  const TermDev* T = &a.terms[it.term];
+template <class T>
+struct KMats {
+  const T* mats;
+  const T* V;
+  const T* ck;
+};
+
+template <class T, bool BIG>
+__device__ __forceinline__ void term_t1(const KArgs& a, const KMats<T>& km, const Item& it,
+                                        T* lds) {
+  const TermDev* Td = &a.terms[it.term];
   const int lg = it.lgnv, nv = 1 << lg;
-  const int nact = T->nact;
-  const int32_t* map = a.map + T->boff;
-  const double* V = a.V + T->qoff;
+  const int nact = Td->nact;
+  const int32_t* map = a.map + Td->boff;
+  const T* V = km.V + Td->qoff;
 
   if (nact == 0) {  // the one-point term
     if ((int)threadIdx.x < nv) {
       const int m = map[0];
       if (m >= 0) {
         const int64_t r = (int64_t)m * a.nvec + it.v0 + threadIdx.x;
-        atomicAdd(&a.hpsi[r], T->wgt * (V[0] + T->vshift) * a.psi[r]);
+        atomicAdd(&a.hpsi[r], Td->wgt * (double)(((T)V[0] + (T)Td->vshift) * (T)a.psi[r]));
       }
     }
     return;
   }
 
-  int nq[MAXACT], nb[MAXACT];
-  const MatDev* md[MAXACT];
-  for (int p = 0; p < nact; ++p) {
-    md[p] = &a.md[T->mat[p]];
-    nq[p] = md[p]->nq;
-    nb[p] = md[p]->nb;
-  }
-  double* buf0 = lds;
-  double* buf1 = BIG ? a.scratch + (int64_t)it.slot * a.slot_size : lds + (size_t)T->nq * nv;
+  T* buf0 = lds;
+  T* buf1 = BIG ? (T*)a.scratch + (int64_t)it.slot * a.slot_size : lds + (size_t)Td->nq * nv;
 
-  PassCtx c;
+  PassCtx<T> c;
   c.map = map; c.psi = a.psi; c.hpsi = a.hpsi; c.nvec = a.nvec; c.v0 = it.v0;
-  c.wgt = T->wgt; c.V = V; c.vshift = T->vshift; c.lgnv = lg; c.coef = 1.0;
+  c.wgt = Td->wgt; c.V = V; c.vshift = (T)Td->vshift; c.lgnv = lg; c.coef = 1;
 
   // ---- basis -> grid ----
-  int A = 1;            // product of already transformed sizes
-  int Bafter = T->nb;   // product of untransformed sizes (incl. current)
-  double* cur = nullptr;
-  double* nxt = buf0;
+  int A = 1;             // product of already transformed sizes
+  int Bafter = Td->nb;   // product of untransformed sizes (incl. current)
+  T* cur = nullptr;
+  T* nxt = buf0;
   for (int p = 0; p < nact; ++p) {
-    Bafter /= nb[p];
-    c.M = a.mats + md[p]->B;
+    const MatDev* md = &a.md[Td->mat[p]];
+    const int nq = md->nq, nb = md->nb;
+    Bafter /= nb;
+    c.MR = km.mats + md->B;
     c.in = cur; c.out = nxt;
     c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
     c.nfib = A * nv * Bafter;
-    if (p == 0) dispatch_pass<IN_GATHER, OUT_STORE>(c, nb[p], nq[p]);
-    else dispatch_pass<IN_LDS, OUT_STORE>(c, nb[p], nq[p]);
+    if (p == 0) dispatch_pass<T, !BIG, IN_GATHER, OUT_STORE>(c, nb, nq);
+    else dispatch_pass<T, !BIG, IN_LDS, OUT_STORE>(c, nb, nq);
     __syncthreads();
-    A *= nq[p];
+    A *= nq;
     cur = nxt;
     nxt = (nxt == buf0) ? buf1 : buf0;
   }
   // ---- grid: V psi + sum_k c_k d2_k psi ----
-  double* pg = cur;
-  double* acc = nxt;
+  T* pg = cur;
+  T* acc = nxt;
   {
-    int Ab = 1, Aa = T->nq;
+    int Ab = 1, Aa = Td->nq;
     for (int p = 0; p < nact; ++p) {
-      Aa /= nq[p];
-      c.M = a.mats + md[p]->D2;
+      const MatDev* md = &a.md[Td->mat[p]];
+      const int nq = md->nq;
+      Aa /= nq;
+      c.MR = km.mats + md->D2;
       c.in = pg; c.out = acc;
       c.A = Ab; c.AV = Ab * nv; c.invAV = 1.0f / (float)c.AV;
       c.nfib = Ab * nv * Aa;
-      c.coef = a.ck[T->kmode[p]];
-      if (p == 0) dispatch_pass<IN_LDS, OUT_VSTORE>(c, nq[p], nq[p]);
-      else dispatch_pass<IN_LDS, OUT_ADD>(c, nq[p], nq[p]);
+      c.coef = km.ck[Td->kmode[p]];
+      if (p == 0) dispatch_pass<T, !BIG, IN_LDS, OUT_VSTORE>(c, nq, nq);
+      else dispatch_pass<T, !BIG, IN_LDS, OUT_ADD>(c, nq, nq);
       __syncthreads();
-      Ab *= nq[p];
+      Ab *= nq;
     }
   }
   // ---- grid -> basis, last pass scatters ----
-  c.coef = 1.0;
+  c.coef = 1;
   cur = acc;
   nxt = pg;
   A = 1;
-  int Aq = T->nq;
+  int Aq = Td->nq;
   for (int p = 0; p < nact; ++p) {
-    Aq /= nq[p];
-    c.M = a.mats + md[p]->BtW;
+    const MatDev* md = &a.md[Td->mat[p]];
+    const int nq = md->nq, nb = md->nb;
+    Aq /= nq;
+    c.MR = km.mats + md->BtW;
     c.in = cur; c.out = nxt;
     c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
     c.nfib = A * nv * Aq;
     if (p == nact - 1) {
-      dispatch_pass<IN_LDS, OUT_SCATTER>(c, nq[p], nb[p]);
+      dispatch_pass<T, !BIG, IN_LDS, OUT_SCATTER>(c, nq, nb);
     } else {
-      dispatch_pass<IN_LDS, OUT_STORE>(c, nq[p], nb[p]);
+      dispatch_pass<T, !BIG, IN_LDS, OUT_STORE>(c, nq, nb);
       __syncthreads();
     }
-    A *= nb[p];
-    double* t = cur; cur = nxt; nxt = t;
+    A *= nb;
+    T* t = cur; cur = nxt; nxt = t;
   }
 }
 
-__global__ __launch_bounds__(NT) void k_hpsi_t1_f64(KArgs a) {
+__global__ __launch_bounds__(NT) void k_hpsi_t1_f64(KArgs a, KMats<double> km) {
   extern __shared__ double lds[];
   const Item it = a.items[blockIdx.x];
-  if (it.big) term_t1_f64<true>(a, it, lds);
-  else term_t1_f64<false>(a, it, lds);
+  if (it.big) term_t1<double, true>(a, km, it, lds);
+  else term_t1<double, false>(a, km, it, lds);
+}
+
+__global__ __launch_bounds__(NT) void k_hpsi_t1_f32(KArgs a, KMats<float> km) {
+  extern __shared__ double lds_d[];
+  float* lds = (float*)lds_d;
+  const Item it = a.items[blockIdx.x];
+  if (it.big) term_t1<float, true>(a, km, it, lds);
+  else term_t1<float, false>(a, km, it, lds);
+}
+
+__global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = (float)in[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -430,6 +471,9 @@ struct sg4_plan {
   bool have_nodes = false;
   sg4::DevBuf d_mats, d_md, d_terms, d_map, d_V, d_ck, d_nodes, d_node_off, d_nqtab,
       d_levels, d_qoff;
+  sg4::DevBuf d_mats32, d_V32, d_ck32;   // FP32 copies for the FP32 modes
+  bool f32_valid = false;
+  int64_t nmats = 0;
   sg4::DevBuf d_rows_in, d_rows_out, d_cols_in, d_cols_out;
   std::map<int, std::unique_ptr<sg4::ItemPlan>> items;  // per nvec
   double flops_per_vec = 0, bytes_fixed = 0;
@@ -492,8 +536,9 @@ static int build_terms(sg4_plan* P) {
 // term too large for that even with one vector keeps its second buffer in a
 // global scratch slot.  Items are ordered by decreasing work so that the
 // dispatcher starts the long ones first.
-static int get_items(sg4_plan* P, int nvec, ItemPlan** out) {
-  auto itp = P->items.find(nvec);
+static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out) {
+  const int key = nvec * 16 + esize;
+  auto itp = P->items.find(key);
   if (itp != P->items.end()) { *out = itp->second.get(); return SG4_OK; }
   struct HI { Item it; double cost; };
   std::vector<HI> all;
@@ -503,9 +548,9 @@ static int get_items(sg4_plan* P, int nvec, ItemPlan** out) {
   for (int64_t t = 0; t < P->nterms; ++t) {
     const TermDev& T = P->h_terms[t];
     const int64_t nmax = std::max(T.nq, T.nb);
-    const int64_t per = 2 * nmax * 8;  // bytes per vector (two buffers)
+    const int64_t per = 2 * nmax * esize;  // bytes per vector (two buffers)
     const bool big = per > LDS_SMALL;
-    if (big && nmax * 8 > LDS_SMALL) {
+    if (big && nmax * esize > LDS_SMALL) {
       set_error("term with " + std::to_string(nmax) + " grid points exceeds the LDS capacity");
       return SG4_ERR_UNSUPPORTED;
     }
@@ -525,7 +570,7 @@ static int get_items(sg4_plan* P, int nvec, ItemPlan** out) {
       h.it.lgnv = (int8_t)g;
       h.it.big = big ? 1 : 0;
       h.it.slot = big ? (int32_t)(nbig++) : 0;
-      lds = std::max<int>(lds, (int)(big ? nmax * 8 : per << g));
+      lds = std::max<int>(lds, (int)(big ? nmax * esize : per << g));
       slot = std::max<int64_t>(slot, big ? nmax : 0);
       h.cost = work * (1 << g);
       all.push_back(h);
@@ -543,9 +588,9 @@ static int get_items(sg4_plan* P, int nvec, ItemPlan** out) {
   int rc = ip->items.alloc(sizeof(Item) * h.size());
   if (rc) return rc;
   HIP_TRY(hipMemcpy(ip->items.p, h.data(), sizeof(Item) * h.size(), hipMemcpyHostToDevice));
-  if (nbig > 0 && (rc = ip->scratch.alloc(sizeof(double) * nbig * slot))) return rc;
+  if (nbig > 0 && (rc = ip->scratch.alloc((size_t)esize * nbig * slot))) return rc;
   *out = ip.get();
-  P->items[nvec] = std::move(ip);
+  P->items[key] = std::move(ip);
   return SG4_OK;
 }
 
@@ -554,23 +599,42 @@ static int grid_blocks(int64_t n) {
   return (int)std::min<int64_t>(std::max<int64_t>(b, 1), 8192);
 }
 
+static int ensure_f32(sg4_plan* P) {
+  if (P->f32_valid) return SG4_OK;
+  int rc;
+  if (!P->d_mats32.p && (rc = P->d_mats32.alloc(sizeof(float) * P->nmats))) return rc;
+  if (!P->d_V32.p && (rc = P->d_V32.alloc(sizeof(float) * P->sum_nq))) return rc;
+  if (!P->d_ck32.p && (rc = P->d_ck32.alloc(sizeof(float) * P->D))) return rc;
+  hipLaunchKernelGGL(k_to_f32, dim3(grid_blocks(P->nmats)), dim3(256), 0, P->stream,
+                     P->d_mats.as<double>(), P->d_mats32.as<float>(), P->nmats);
+  hipLaunchKernelGGL(k_to_f32, dim3(grid_blocks(P->sum_nq)), dim3(256), 0, P->stream,
+                     P->d_V.as<double>(), P->d_V32.as<float>(), P->sum_nq);
+  hipLaunchKernelGGL(k_to_f32, dim3(1), dim3(64), 0, P->stream, P->d_ck.as<double>(),
+                     P->d_ck32.as<float>(), (int64_t)P->D);
+  HIP_TRY(hipGetLastError());
+  P->f32_valid = true;
+  return SG4_OK;
+}
+
 static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, int prec) {
   if (P->type_op == 0) { set_error("no operator set on the plan"); return SG4_ERR_STATE; }
   if (nvec < 1 || nvec > P->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
-  if (prec != SG4_PREC_FP64) { set_error("precision mode not available"); return SG4_ERR_UNSUPPORTED; }
+  if (prec != SG4_PREC_FP64 && prec != SG4_PREC_F32) {
+    set_error("precision mode not available");
+    return SG4_ERR_UNSUPPORTED;
+  }
   if (P->type_op != 1) { set_error("operator type not available"); return SG4_ERR_UNSUPPORTED; }
+  const int esize = (prec == SG4_PREC_FP64) ? 8 : 4;
+  int rc;
+  if (esize == 4 && (rc = ensure_f32(P))) return rc;
   ItemPlan* ip = nullptr;
-  int rc = get_items(P, nvec, &ip);
-  if (rc) return rc;
+  if ((rc = get_items(P, nvec, esize, &ip))) return rc;
   const int64_t n = P->nb_ba * nvec;
   hipLaunchKernelGGL(k_fill, dim3(grid_blocks(n)), dim3(256), 0, P->stream, hpsi, n, 0.0);
   KArgs a;
   a.terms = P->d_terms.as<TermDev>();
   a.md = P->d_md.as<MatDev>();
-  a.mats = P->d_mats.as<double>();
   a.map = P->d_map.as<int32_t>();
-  a.V = P->d_V.as<double>();
-  a.ck = P->d_ck.as<double>();
   a.psi = psi;
   a.hpsi = hpsi;
   a.nvec = nvec;
@@ -589,7 +653,13 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
     ++P->ev_used;
     HIP_TRY(hipEventRecord(P->ev0, P->stream));
   }
-  hipLaunchKernelGGL(k_hpsi_t1_f64, dim3((unsigned)ip->nitems), dim3(NT), ip->lds, P->stream, a);
+  if (esize == 8) {
+    KMats<double> km{P->d_mats.as<double>(), P->d_V.as<double>(), P->d_ck.as<double>()};
+    hipLaunchKernelGGL(k_hpsi_t1_f64, dim3((unsigned)ip->nitems), dim3(NT), ip->lds, P->stream, a, km);
+  } else {
+    KMats<float> km{P->d_mats32.as<float>(), P->d_V32.as<float>(), P->d_ck32.as<float>()};
+    hipLaunchKernelGGL(k_hpsi_t1_f32, dim3((unsigned)ip->nitems), dim3(NT), ip->lds, P->stream, a, km);
+  }
   if (P->timing) HIP_TRY(hipEventRecord(P->ev1, P->stream));
   HIP_TRY(hipGetLastError());
   return SG4_OK;
@@ -673,6 +743,8 @@ int sg4_plan_create(const sg4_tables* t, int32_t max_nvec, sg4_plan** out) {
   std::call_once(attr_once, [] {
     (void)hipFuncSetAttribute((const void*)k_hpsi_t1_f64,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
+    (void)hipFuncSetAttribute((const void*)k_hpsi_t1_f32,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
   });
   std::unique_ptr<sg4_plan> P(new sg4_plan);
   P->D = t->D; P->Lmax = t->Lmax; P->nterms = t->nterms; P->nb_ba = t->nb_ba;
@@ -724,8 +796,22 @@ int sg4_plan_create(const sg4_tables* t, int32_t max_nvec, sg4_plan** out) {
   for (int64_t e = 0; e < P->sum_nb; ++e)
     if (t->map[e] < -1 || t->map[e] >= t->nb_ba) { set_error("map entry out of range"); return SG4_ERR_ARG; }
   int rc;
+  P->nmats = off;
+  // the kernels read the 1-D matrices row by row: store row-major copies at
+  // the same offsets
+  std::vector<double> matsR(off);
+  for (const MatDev& m : P->h_md) {
+    auto tr = [&](int64_t o, int rows, int cols) {
+      for (int i = 0; i < rows; ++i)
+        for (int j = 0; j < cols; ++j) matsR[o + (int64_t)i * cols + j] = t->mats[o + i + (int64_t)rows * j];
+    };
+    tr(m.B, m.nq, m.nb);
+    tr(m.BtW, m.nb, m.nq);
+    tr(m.D1, m.nq, m.nq);
+    tr(m.D2, m.nq, m.nq);
+  }
   if ((rc = P->d_mats.alloc(sizeof(double) * off))) return rc;
-  HIP_TRY(hipMemcpy(P->d_mats.p, t->mats, sizeof(double) * off, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(P->d_mats.p, matsR.data(), sizeof(double) * off, hipMemcpyHostToDevice));
   if ((rc = P->d_md.alloc(sizeof(MatDev) * P->h_md.size()))) return rc;
   HIP_TRY(hipMemcpy(P->d_md.p, P->h_md.data(), sizeof(MatDev) * P->h_md.size(), hipMemcpyHostToDevice));
   if ((rc = P->d_map.alloc(sizeof(int32_t) * P->sum_nb))) return rc;
@@ -778,6 +864,7 @@ int sg4_plan_set_operator(sg4_plan* p, const sg4_operator* op) {
   HIP_TRY(hipMemcpy(p->d_ck.p, op->ck, sizeof(double) * p->D, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(p->d_V.p, op->V, sizeof(double) * p->sum_nq, hipMemcpyHostToDevice));
   p->items.clear();
+  p->f32_valid = false;
   return build_terms(p);
 }
 
@@ -801,6 +888,7 @@ int sg4_plan_set_henon_heiles(sg4_plan* p, double lambda) {
   if ((rc = p->d_ck.alloc(sizeof(double) * p->D))) return rc;
   HIP_TRY(hipMemcpy(p->d_ck.p, p->h_ck.data(), sizeof(double) * p->D, hipMemcpyHostToDevice));
   p->items.clear();
+  p->f32_valid = false;
   rc = build_terms(p);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(p->stream));
@@ -851,8 +939,8 @@ int sg4_plan_stats(const sg4_plan* p, int32_t nvec, double* flops, double* bytes
   if (flops) *flops = p->flops_per_vec * nvec;
   if (bytes) *bytes = p->bytes_fixed + 16.0 * (double)p->nb_ba * nvec;
   if (items) {
-    auto it = p->items.find(nvec);
-    *items = (it == p->items.end()) ? -1 : it->second->nitems;
+    auto it8 = p->items.find(nvec * 16 + 8);
+    *items = (it8 == p->items.end()) ? -1 : it8->second->nitems;
   }
   return SG4_OK;
 }

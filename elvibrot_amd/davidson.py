@@ -1,0 +1,167 @@
+"""Block Davidson on the device around the SG4 H.psi library.
+
+Mirrors the control flow of sub_propagation_Davidson
+(SRC/sub_propagation/sub_module_Davidson.f90:59-763):
+
+* each iteration applies H to the block of new vectors (sub_MakeHPsi_Davidson,
+  :896-1040, block call :970), updates the projected matrix
+  (sub_MakeH_Davidson, :1044-1159) and diagonalises it (:362);
+* convergence of state j needs both |E_j - E_j(previous)| < conv_ene and
+  ||r_j|| < conv_resi (:420-473);
+* new vectors are the residuals r_j = sum_i (H psi_i - E_j psi_i) Vec(i,j)
+  (MakeResidual_Davidson, :1230-1304) divided by E_j - E0(ib), with the
+  reference's guard 1/(D + 1e-3) when |D| <= conv_resi (NewVec_type = 4,
+  sub_NewVec_Davidson :1440-1460), then normalised and orthogonalised twice
+  against the basis (:1529-1547); vectors with norm^2 < 1e-10 are dropped;
+* when the basis would exceed max_WP it is collapsed onto the current Ritz
+  vectors (restart).
+
+All block algebra (projections, residuals, Gram-Schmidt, norms) is FP64 on
+the device; the small projected eigenproblem is solved on the host.  With
+``prec`` = PREC_F32 the basis is expanded with the FP32 H.psi; the converged
+block then gets one FP64 Rayleigh-Ritz (an FP64 H.psi on the Ritz vectors),
+which makes the eigenvalue error second order in the FP32 error.
+"""
+import numpy as np
+import torch
+
+from . import PREC_FP64
+
+
+def ho_zeroth_order_diagonal(basis, device):
+    """E0(ib) = sum_k (i_k + 1/2) scaleQ-independent HO energies of the packed
+    basis functions (the role of get_Ene0_AT_ib_FROM_Basis,
+    SRC/sub_Basis/sub_module_basis_set_alloc.f90:3122, for HO primitive bases)."""
+    lev = basis.levels()
+    mp = basis.map()
+    nb = basis.nb_ba
+    e0 = np.full(nb, np.nan)
+    boff = 0
+    for t in range(basis.nterms):
+        dims = [basis.basis1d(k, int(lev[t, k]))["nb"] for k in range(basis.D)]
+        n = int(np.prod(dims))
+        idx = np.indices(dims[::-1]).reshape(basis.D, -1)[::-1]   # first index fastest
+        rows = mp[boff:boff + n]
+        ok = rows >= 0
+        e0[rows[ok]] = idx[:, ok].sum(0) + 0.5 * basis.D
+        boff += n
+    return torch.tensor(e0, device=device, dtype=torch.float64)
+
+
+class DavidsonResult:
+    def __init__(self, E, X, it, n_hpsi, converged, E_fp64=None, history=None):
+        self.E, self.X, self.it, self.n_hpsi = E, X, it, n_hpsi
+        self.converged, self.E_fp64, self.history = converged, E_fp64, history or []
+
+
+def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
+                             conv_ene=1e-4, block=None, prec=PREC_FP64, psi0=None,
+                             final_rr_fp64=True, device="cuda:0", verbose=False):
+    """Lowest ``nb_WP`` eigenpairs of the SG4 operator ``op``.
+
+    Returns a DavidsonResult with E (host numpy), X (device, nb_WP x nb_ba),
+    iterations, number of H.psi vector applications and, when ``prec`` is not
+    FP64, the FP64 Rayleigh-Ritz eigenvalues E_fp64 (E is then E_fp64)."""
+    nb = op.nb_ba
+    block = block or min(op.max_nvec, nb_WP)
+    max_WP = max_WP or max(4 * nb_WP, nb_WP + 4 * block)
+    max_WP = min(max_WP, nb)
+    dev = torch.device(device)
+    E0 = ho_zeroth_order_diagonal(op.basis, dev)
+    Vb = torch.zeros(max_WP, nb, device=dev, dtype=torch.float64)   # basis (rows = vectors)
+    HV = torch.zeros_like(Vb)
+    if psi0 is None:
+        order = torch.argsort(E0)[:nb_WP]
+        psi0 = torch.zeros(nb_WP, nb, device=dev, dtype=torch.float64)
+        psi0[torch.arange(nb_WP, device=dev), order] = 1.0
+    new = psi0
+    ndim = 0
+    n_hpsi = 0
+    Ene0 = None
+    E = X = None
+    conv = np.zeros(nb_WP, dtype=bool)
+    hist = []
+
+    def apply(blk, p):
+        nonlocal n_hpsi
+        out = torch.empty_like(blk)
+        for a in range(0, blk.shape[0], op.max_nvec):
+            b = min(a + op.max_nvec, blk.shape[0])
+            op.apply(blk[a:b].contiguous(), out[a:b], p)
+        n_hpsi += blk.shape[0]
+        return out
+
+    def orthonormalize(W, ndim):
+        keep = []
+        for j in range(W.shape[0]):
+            w = W[j]
+            w = w / torch.linalg.norm(w)
+            for _ in range(2):   # double Gram-Schmidt against the basis and accepted new vectors
+                if ndim:
+                    w = w - Vb[:ndim].T @ (Vb[:ndim] @ w)
+                for q in keep:
+                    w = w - q * torch.dot(q, w)
+                nrm2 = torch.dot(w, w).item()
+                if nrm2 < 1e-10:
+                    w = None
+                    break
+                w = w / np.sqrt(nrm2)
+            if w is not None:
+                keep.append(w)
+        return torch.stack(keep) if keep else None
+
+    it = 0
+    for it in range(max_it + 1):
+        W = orthonormalize(new, ndim)
+        if W is None:
+            break
+        m = W.shape[0]
+        if ndim + m > max_WP:          # restart on the current Ritz vectors
+            keep = min(max(nb_WP + block, nb_WP), max_WP - m)
+            Vb[:keep] = X_all[:keep]
+            HV[:keep] = HX_all[:keep]
+            ndim = keep
+            W = orthonormalize(new, ndim)
+            if W is None:
+                break
+            m = W.shape[0]
+        Vb[ndim:ndim + m] = W
+        HV[ndim:ndim + m] = apply(W, prec)
+        ndim += m
+        Hp = Vb[:ndim] @ HV[:ndim].T
+        Hp = 0.5 * (Hp + Hp.T)
+        ev, vec = np.linalg.eigh(Hp.cpu().numpy())
+        vec_t = torch.tensor(vec, device=dev)
+        X_all = vec_t.T @ Vb[:ndim]
+        HX_all = vec_t.T @ HV[:ndim]
+        E = ev[:nb_WP]
+        X = X_all[:nb_WP]
+        R = HX_all[:nb_WP] - torch.tensor(E, device=dev)[:, None] * X
+        rn = torch.linalg.norm(R, dim=1).cpu().numpy()
+        dE = np.abs(E - Ene0) if Ene0 is not None else np.full(nb_WP, np.inf)
+        conv = (rn < conv_resi) & (dE < conv_ene)
+        hist.append((it, ndim, float(rn.max()), float(dE.max())))
+        if verbose:
+            print(f"davidson it {it} ndim {ndim} max|r| {rn.max():.2e} max|dE| {dE.max():.2e} conv {conv.sum()}")
+        Ene0 = E.copy()
+        if conv.all():
+            break
+        todo = np.nonzero(~conv)[0][:block]
+        Rt = R[todo]
+        Et = torch.tensor(E[todo], device=dev)[:, None]
+        Dg = Et - E0[None, :]
+        Dg = torch.where(Dg.abs() > conv_resi, Dg, Dg + 1e-3)
+        new = Rt / Dg
+    E_fp64 = None
+    if prec != PREC_FP64 and final_rr_fp64 and X is not None:
+        nr = min(ndim, nb_WP + block)
+        Q = X_all[:nr]
+        Q, _ = torch.linalg.qr(Q.T)
+        Q = Q.T.contiguous()
+        HQ = apply(Q, PREC_FP64)
+        Hq = Q @ HQ.T
+        ev, vec = np.linalg.eigh(0.5 * (Hq + Hq.T).cpu().numpy())
+        E_fp64 = ev[:nb_WP]
+        X = (torch.tensor(vec[:, :nb_WP], device=dev).T @ Q)
+        E = E_fp64
+    return DavidsonResult(E, X, it, n_hpsi, conv, E_fp64, hist)

@@ -1,0 +1,44 @@
+"""Device block Davidson (FP64 block algebra, SG4 H.psi through the C ABI)
+against exact eigenvalues of the same discrete operator (dense diagonalisation
+of the oracle matrix) and the reference's benchmark levels."""
+import os
+
+import numpy as np
+import pytest
+
+import elvibrot_amd as E
+from elvibrot_amd.davidson import sub_propagation_Davidson
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def exact_levels(D, L, n):
+    o = O.SG4(D, L)
+    H = o.hpsi(np.eye(o.nb_ba), o.grid_hh(0.111803), -0.5 * np.ones(D))
+    return np.linalg.eigvalsh(0.5 * (H + H.T))[:n]
+
+
+@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_F32])
+def test_davidson_6d_L3_28_levels(cuda, golden, prec):
+    b = E.SG4Basis(6, 3)
+    op = E.SG4Operator(b, max_nvec=16)
+    r = sub_propagation_Davidson(op, 28, max_it=300, conv_resi=1e-6, conv_ene=1e-10, prec=prec)
+    ex = exact_levels(6, 3, 28)
+    err = np.max(np.abs(r.E - ex) / np.abs(ex))
+    print(f"prec {prec}: it {r.it} H.psi {r.n_hpsi} max rel err {err:.2e}")
+    assert err < 1e-8
+    ref = np.loadtxt(os.path.join(golden, "hh6d_L3_benchmark.txt"))[:, 0]
+    assert np.max(np.abs(r.E - ref)) < 1.5e-7
+
+
+def test_davidson_6d_L6_20_levels_f32_with_fp64_rayleigh_ritz(cuda):
+    """FP32 expansion, FP64 Rayleigh-Ritz: within 1e-8 of the FP64 Davidson."""
+    b = E.SG4Basis(6, 6)
+    op = E.SG4Operator(b, max_nvec=16)
+    r64 = sub_propagation_Davidson(op, 20, max_it=400, conv_resi=1e-6, conv_ene=1e-10)
+    r32 = sub_propagation_Davidson(op, 20, max_it=400, conv_resi=1e-5, conv_ene=1e-8,
+                                   prec=E.PREC_F32)
+    err = np.max(np.abs(r32.E - r64.E) / np.abs(r64.E))
+    print(f"f32+RR vs fp64: {err:.2e} (it {r32.it}, {r32.n_hpsi} H.psi)")
+    assert err < 1e-8

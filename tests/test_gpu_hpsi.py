@@ -118,3 +118,19 @@ def test_errors_are_reported(cuda):
         op.apply_host(np.zeros((b.nb_ba, 3)))
     with pytest.raises(E.SG4Error):
         op.apply_host(np.zeros((b.nb_ba, 1)), prec=7)
+
+
+@pytest.mark.parametrize("D,L", [(6, 6), (10, 5)])
+def test_f32_mode_accuracy(cuda, D, L):
+    """FP32 mode contractions (FP64 gather/scatter): H.psi within 1e-5
+    relative to max|H psi| of the FP64 oracle (measured: a few 1e-7; the
+    tolerance leaves room for the roundoff amplification of the Smolyak
+    combination)."""
+    b, op, o, V = hh_pair(D, L)
+    psi = np.random.default_rng(99).standard_normal((b.nb_ba, 4))
+    ref = o.hpsi(psi, V, -0.5 * np.ones(D))
+    got = op.apply_host(psi, prec=E.PREC_F32)
+    err = rel(got, ref)
+    print(f"f32 rel err {err:.2e}")
+    assert err < 1e-5
+    assert err > 1e-12   # it really ran in reduced precision
