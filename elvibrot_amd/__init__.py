@@ -114,9 +114,10 @@ class SG4Operator:
     """
 
     def __init__(self, basis, max_nvec=16, potential=("henon_heiles", 0.111803), ck=None,
-                 stream=None):
+                 stream=None, type_op=1, G=None, Jac=None, sqRJ=None):
         self.basis = basis
         self.max_nvec = max_nvec
+        self.type_op = type_op
         h = C.c_void_p()
         _capi.check(lib().sg4_plan_create(C.byref(basis.tables), max_nvec, C.byref(h)))
         self._h = h
@@ -124,17 +125,29 @@ class SG4Operator:
             _capi.check(lib().sg4_plan_set_stream(h, C.c_void_p(stream)))
         if isinstance(potential, tuple) and potential[0] == "henon_heiles":
             _capi.check(lib().sg4_plan_set_henon_heiles(h, float(potential[1])))
-        else:
-            V = np.ascontiguousarray(potential, dtype=np.float64)
-            if V.shape != (basis.sum_nq,):
-                raise ValueError("V must have basis.sum_nq entries")
-            ckv = np.ascontiguousarray(-0.5 * np.ones(basis.D) if ck is None else ck,
-                                       dtype=np.float64)
-            op = _capi.Operator()
-            op.type_op = 1
-            op.V = V.ctypes.data_as(_capi.D_P)
+            return
+        c64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        V = c64(potential)
+        if V.shape != (basis.sum_nq,):
+            raise ValueError("V must have basis.sum_nq entries")
+        op = _capi.Operator()
+        op.type_op = type_op
+        op.V = V.ctypes.data_as(_capi.D_P)
+        keep = [V]
+        if type_op == 1:
+            ckv = c64(-0.5 * np.ones(basis.D) if ck is None else ck)
             op.ck = ckv.ctypes.data_as(_capi.D_P)
-            _capi.check(lib().sg4_plan_set_operator(h, C.byref(op)))
+            keep.append(ckv)
+        else:
+            G, Jac, sqRJ = c64(G), c64(Jac), c64(sqRJ)
+            if G.size != basis.sum_nq * basis.D ** 2 or Jac.size != basis.sum_nq or \
+                    sqRJ.size != basis.sum_nq:
+                raise ValueError("G, Jac, sqRJ must be given on every grid point")
+            op.G = G.ctypes.data_as(_capi.D_P)
+            op.Jac = Jac.ctypes.data_as(_capi.D_P)
+            op.sqRJ = sqRJ.ctypes.data_as(_capi.D_P)
+            keep += [G, Jac, sqRJ]
+        _capi.check(lib().sg4_plan_set_operator(h, C.byref(op)))
 
     def __del__(self):
         if getattr(self, "_h", None):

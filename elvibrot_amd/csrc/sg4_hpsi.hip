@@ -225,7 +225,7 @@ __device__ __forceinline__ void dispatch_pass(const PassCtx<T>& c, int NI, int N
   if (FAST && NI == NO) {
     switch (NI) {
 #define SG4_CASE(n) case n: mode_pass<T, n, n, KIN, KOUT>(c); return;
-      SG4_CASE(2) SG4_CASE(3) SG4_CASE(4) SG4_CASE(5) SG4_CASE(6) SG4_CASE(7)
+      SG4_CASE(1) SG4_CASE(2) SG4_CASE(3) SG4_CASE(4) SG4_CASE(5) SG4_CASE(6) SG4_CASE(7)
       SG4_CASE(8) SG4_CASE(9) SG4_CASE(10) SG4_CASE(11) SG4_CASE(12) SG4_CASE(13)
       SG4_CASE(14) SG4_CASE(15) SG4_CASE(16) SG4_CASE(17) SG4_CASE(18) SG4_CASE(19)
       SG4_CASE(20) SG4_CASE(21) SG4_CASE(22) SG4_CASE(23) SG4_CASE(24)
@@ -355,6 +355,134 @@ __global__ __launch_bounds__(NT) void k_hpsi_t1_f32(KArgs a, KMats<float> km) {
   else term_t1<float, false>(a, km, it, lds);
 }
 
+// ---------------------------------------------------------------------------
+// Type-10 operator (curvilinear kinetic d1 G d1), FP64
+// (sub_TabOpPsi_OF_ONEDP_FOR_SGtype4 case 10, SRC/sub_Operator/sub_OpPsi_SG4.f90
+// :1546-1649):
+//   u = sqRJ psi ; u_j = d1_j u ; w_i = Jac sum_j G(j,i) u_j ;
+//   H psi = -1/2 sum_i d1_i w_i / (Jac sqRJ) + V psi
+// Every mode takes part (a one-point mode has a 1x1 d1).  LDS holds D + 3
+// buffers of nq_t x nv: two for the transforms, the kinetic accumulator, and
+// one per derivative u_j.
+// ---------------------------------------------------------------------------
+struct KMats10 {
+  const double* mats;
+  const double* V;
+  const double* G;
+  const double* Jac;
+  const double* sqRJ;
+};
+
+__global__ __launch_bounds__(NT) void k_hpsi_t10_f64(KArgs a, KMats10 km) {
+  extern __shared__ double lds[];
+  const Item it = a.items[blockIdx.x];
+  const TermDev* Td = &a.terms[it.term];
+  const int lg = it.lgnv, nv = 1 << lg;
+  const int D = Td->nact;
+  const size_t SZ = (size_t)Td->nq * nv;
+  double* b0 = lds;
+  double* b1 = lds + SZ;
+  double* O = lds + 2 * SZ;
+  double* U = lds + 3 * SZ;
+  const int64_t q0 = Td->qoff;
+
+  PassCtx<double> c;
+  c.map = a.map + Td->boff; c.psi = a.psi; c.hpsi = a.hpsi; c.nvec = a.nvec; c.v0 = it.v0;
+  c.wgt = Td->wgt; c.V = km.V + q0; c.vshift = 0.0; c.lgnv = lg; c.coef = 1.0;
+
+  // ---- basis -> grid ----
+  int A = 1, Bafter = Td->nb;
+  double* cur = nullptr;
+  double* nxt = b0;
+  for (int p = 0; p < D; ++p) {
+    const MatDev* md = &a.md[Td->mat[p]];
+    const int nq = md->nq, nb = md->nb;
+    Bafter /= nb;
+    c.MR = km.mats + md->B;
+    c.in = cur; c.out = nxt;
+    c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
+    c.nfib = A * nv * Bafter;
+    if (p == 0) dispatch_pass<double, true, IN_GATHER, OUT_STORE>(c, nb, nq);
+    else dispatch_pass<double, true, IN_LDS, OUT_STORE>(c, nb, nq);
+    __syncthreads();
+    A *= nq;
+    cur = nxt;
+    nxt = (nxt == b0) ? b1 : b0;
+  }
+  double* pg = cur;   // psi on the grid
+  double* S = nxt;    // scratch: u, then w_i
+  for (size_t e = threadIdx.x; e < SZ; e += NT) S[e] = km.sqRJ[q0 + (e >> lg)] * pg[e];
+  __syncthreads();
+  // ---- u_j = d1_j u ----
+  {
+    int Ab = 1, Aa = Td->nq;
+    for (int j = 0; j < D; ++j) {
+      const MatDev* md = &a.md[Td->mat[j]];
+      const int nq = md->nq;
+      Aa /= nq;
+      c.MR = km.mats + md->D1;
+      c.in = S; c.out = U + (size_t)j * SZ;
+      c.A = Ab; c.AV = Ab * nv; c.invAV = 1.0f / (float)c.AV;
+      c.nfib = Ab * nv * Aa;
+      dispatch_pass<double, true, IN_LDS, OUT_STORE>(c, nq, nq);
+      Ab *= nq;
+    }
+  }
+  __syncthreads();
+  // ---- sum_i d1_i (Jac sum_j G(j,i) u_j) ----
+  {
+    int Ab = 1, Aa = Td->nq;
+    for (int i = 0; i < D; ++i) {
+      for (size_t e = threadIdx.x; e < SZ; e += NT) {
+        const int64_t q = q0 + (int64_t)(e >> lg);
+        const double* Gq = km.G + (int64_t)D * D * q;
+        double w = 0.0;
+        for (int j = 0; j < D; ++j) w = fma(Gq[j + D * i], U[(size_t)j * SZ + e], w);
+        S[e] = km.Jac[q] * w;
+      }
+      __syncthreads();
+      const MatDev* md = &a.md[Td->mat[i]];
+      const int nq = md->nq;
+      Aa /= nq;
+      c.MR = km.mats + md->D1;
+      c.in = S; c.out = O;
+      c.A = Ab; c.AV = Ab * nv; c.invAV = 1.0f / (float)c.AV;
+      c.nfib = Ab * nv * Aa;
+      if (i == 0) dispatch_pass<double, true, IN_LDS, OUT_STORE>(c, nq, nq);
+      else dispatch_pass<double, true, IN_LDS, OUT_ADD>(c, nq, nq);
+      __syncthreads();
+      Ab *= nq;
+    }
+  }
+  for (size_t e = threadIdx.x; e < SZ; e += NT) {
+    const int64_t q = q0 + (int64_t)(e >> lg);
+    O[e] = -0.5 * O[e] / (km.Jac[q] * km.sqRJ[q]) + km.V[q] * pg[e];
+  }
+  __syncthreads();
+  // ---- grid -> basis, last pass scatters ----
+  cur = O;
+  nxt = S;
+  A = 1;
+  int Aq = Td->nq;
+  for (int p = 0; p < D; ++p) {
+    const MatDev* md = &a.md[Td->mat[p]];
+    const int nq = md->nq, nb = md->nb;
+    Aq /= nq;
+    c.MR = km.mats + md->BtW;
+    c.in = cur; c.out = nxt;
+    c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
+    c.nfib = A * nv * Aq;
+    if (p == D - 1) {
+      dispatch_pass<double, true, IN_LDS, OUT_SCATTER>(c, nq, nb);
+    } else {
+      dispatch_pass<double, true, IN_LDS, OUT_STORE>(c, nq, nb);
+      __syncthreads();
+    }
+    A *= nb;
+    double* t = cur; cur = nxt; nxt = t;
+  }
+}
+
 __global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -472,6 +600,7 @@ struct sg4_plan {
   sg4::DevBuf d_mats, d_md, d_terms, d_map, d_V, d_ck, d_nodes, d_node_off, d_nqtab,
       d_levels, d_qoff;
   sg4::DevBuf d_mats32, d_V32, d_ck32;   // FP32 copies for the FP32 modes
+  sg4::DevBuf d_G, d_Jac, d_sqRJ;        // type-10 grids
   bool f32_valid = false;
   int64_t nmats = 0;
   sg4::DevBuf d_rows_in, d_rows_out, d_cols_in, d_cols_out;
@@ -548,8 +677,13 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out) {
   for (int64_t t = 0; t < P->nterms; ++t) {
     const TermDev& T = P->h_terms[t];
     const int64_t nmax = std::max(T.nq, T.nb);
-    const int64_t per = 2 * nmax * esize;  // bytes per vector (two buffers)
+    const int nbuf = (P->type_op == 10) ? P->D + 3 : 2;
+    const int64_t per = nbuf * nmax * esize;  // bytes per vector (work buffers)
     const bool big = per > LDS_SMALL;
+    if (big && P->type_op == 10) {
+      set_error("type-10 term too large for the LDS work buffers");
+      return SG4_ERR_UNSUPPORTED;
+    }
     if (big && nmax * esize > LDS_SMALL) {
       set_error("term with " + std::to_string(nmax) + " grid points exceeds the LDS capacity");
       return SG4_ERR_UNSUPPORTED;
@@ -623,7 +757,10 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
     set_error("precision mode not available");
     return SG4_ERR_UNSUPPORTED;
   }
-  if (P->type_op != 1) { set_error("operator type not available"); return SG4_ERR_UNSUPPORTED; }
+  if (P->type_op == 10 && prec != SG4_PREC_FP64) {
+    set_error("the type-10 operator is available in FP64 only");
+    return SG4_ERR_UNSUPPORTED;
+  }
   const int esize = (prec == SG4_PREC_FP64) ? 8 : 4;
   int rc;
   if (esize == 4 && (rc = ensure_f32(P))) return rc;
@@ -653,7 +790,11 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
     ++P->ev_used;
     HIP_TRY(hipEventRecord(P->ev0, P->stream));
   }
-  if (esize == 8) {
+  if (P->type_op == 10) {
+    KMats10 km{P->d_mats.as<double>(), P->d_V.as<double>(), P->d_G.as<double>(),
+               P->d_Jac.as<double>(), P->d_sqRJ.as<double>()};
+    hipLaunchKernelGGL(k_hpsi_t10_f64, dim3((unsigned)ip->nitems), dim3(NT), ip->lds, P->stream, a, km);
+  } else if (esize == 8) {
     KMats<double> km{P->d_mats.as<double>(), P->d_V.as<double>(), P->d_ck.as<double>()};
     hipLaunchKernelGGL(k_hpsi_t1_f64, dim3((unsigned)ip->nitems), dim3(NT), ip->lds, P->stream, a, km);
   } else {
@@ -744,6 +885,8 @@ int sg4_plan_create(const sg4_tables* t, int32_t max_nvec, sg4_plan** out) {
     (void)hipFuncSetAttribute((const void*)k_hpsi_t1_f64,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
     (void)hipFuncSetAttribute((const void*)k_hpsi_t1_f32,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
+    (void)hipFuncSetAttribute((const void*)k_hpsi_t10_f64,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
   });
   std::unique_ptr<sg4_plan> P(new sg4_plan);
@@ -841,7 +984,7 @@ int sg4_plan_create(const sg4_tables* t, int32_t max_nvec, sg4_plan** out) {
     const double Nt = (double)(P->h_qoff[tt + 1] - P->h_qoff[tt]);
     for (int k = 0; k < D; ++k) S += 2.0 * Nt * P->h_nq[k * L1 + P->h_levels[(size_t)tt * D + k]];
   }
-  P->flops_per_vec = 3.0 * S;
+  P->flops_per_vec = S;   // sum_t S_t; x3 for type 1, x4 for type 10
   P->bytes_fixed = 8.0 * P->sum_nq + 4.0 * P->sum_nb;
   *out = P.release();
   return SG4_OK;
@@ -855,13 +998,27 @@ int sg4_plan_set_stream(sg4_plan* p, void* s) {
 
 int sg4_plan_set_operator(sg4_plan* p, const sg4_operator* op) {
   if (!p || !op || !op->V) { set_error("null operator"); return SG4_ERR_ARG; }
-  if (op->type_op != 1) { set_error("only type_op = 1 is available"); return SG4_ERR_UNSUPPORTED; }
-  if (!op->ck) { set_error("type 1 needs ck"); return SG4_ERR_ARG; }
-  p->type_op = 1;
-  p->h_ck.assign(op->ck, op->ck + p->D);
   int rc;
-  if ((rc = p->d_ck.alloc(sizeof(double) * p->D))) return rc;
-  HIP_TRY(hipMemcpy(p->d_ck.p, op->ck, sizeof(double) * p->D, hipMemcpyHostToDevice));
+  if (op->type_op == 1) {
+    if (!op->ck) { set_error("type 1 needs ck"); return SG4_ERR_ARG; }
+    p->h_ck.assign(op->ck, op->ck + p->D);
+    if ((rc = p->d_ck.alloc(sizeof(double) * p->D))) return rc;
+    HIP_TRY(hipMemcpy(p->d_ck.p, op->ck, sizeof(double) * p->D, hipMemcpyHostToDevice));
+  } else if (op->type_op == 10) {
+    if (!op->G || !op->Jac || !op->sqRJ) { set_error("type 10 needs G, Jac and sqRJ"); return SG4_ERR_ARG; }
+    const size_t n = (size_t)p->sum_nq;
+    if ((rc = p->d_G.alloc(sizeof(double) * n * p->D * p->D))) return rc;
+    if ((rc = p->d_Jac.alloc(sizeof(double) * n))) return rc;
+    if ((rc = p->d_sqRJ.alloc(sizeof(double) * n))) return rc;
+    HIP_TRY(hipMemcpy(p->d_G.p, op->G, sizeof(double) * n * p->D * p->D, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_Jac.p, op->Jac, sizeof(double) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(p->d_sqRJ.p, op->sqRJ, sizeof(double) * n, hipMemcpyHostToDevice));
+    p->h_ck.assign(p->D, 0.0);
+  } else {
+    set_error("type_op must be 1 or 10");
+    return SG4_ERR_UNSUPPORTED;
+  }
+  p->type_op = op->type_op;
   HIP_TRY(hipMemcpy(p->d_V.p, op->V, sizeof(double) * p->sum_nq, hipMemcpyHostToDevice));
   p->items.clear();
   p->f32_valid = false;
@@ -936,7 +1093,7 @@ int sg4_apply_host(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, i
 
 int sg4_plan_stats(const sg4_plan* p, int32_t nvec, double* flops, double* bytes, int64_t* items) {
   if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
-  if (flops) *flops = p->flops_per_vec * nvec;
+  if (flops) *flops = (p->type_op == 10 ? 4.0 : 3.0) * p->flops_per_vec * nvec;
   if (bytes) *bytes = p->bytes_fixed + 16.0 * (double)p->nb_ba * nvec;
   if (items) {
     auto it8 = p->items.find(nvec * 16 + 8);

@@ -67,3 +67,16 @@ def test_setup_rejects_bad_input():
         E.SG4Basis(3, 2, LB=5)
     with pytest.raises(E.SG4Error):
         E.SG4Basis(2, 3, modes=[dict(nq_A=1, nq_B=1, nb_A=2, nb_B=1)] * 2)
+
+
+def test_setup_jacobi_bases_equal_oracle():
+    import elvibrot_amd.potentials as P
+    b = P.jacobi_triatomic_basis(6)
+    sc = np.sqrt(P.AMU * np.sqrt(0.4 / P.AMU))
+    o = O.SG4(3, 6, kind=[0, 0, 1], Q0=[3.0, 2.0, 0.0], scaleQ=[sc, sc, 1.0], nq_A=2, nb_A=2)
+    assert np.array_equal(b.map(), o.map)
+    for k in range(3):
+        for l in range(7):
+            bb, ob = b.basis1d(k, l), o.basis1d(k, l)
+            for key in ("x", "w", "d0", "d1", "d2"):
+                assert np.max(np.abs(bb[key] - ob[key])) <= 1e-13 * max(1, np.abs(ob[key]).max())

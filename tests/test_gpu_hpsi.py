@@ -134,3 +134,17 @@ def test_f32_mode_accuracy(cuda, D, L):
     print(f"f32 rel err {err:.2e}")
     assert err < 1e-5
     assert err > 1e-12   # it really ran in reduced precision
+
+
+@pytest.mark.parametrize("L,nvec", [(4, 3), (10, 5)])
+def test_type10_jacobi_parity(cuda, L, nvec):
+    import elvibrot_amd.potentials as P
+    op = P.jacobi_triatomic_operator(L, max_nvec=8)
+    b = op.basis
+    sc = np.sqrt(P.AMU * np.sqrt(0.4 / P.AMU))
+    o = O.SG4(3, L, kind=[0, 0, 1], Q0=[3.0, 2.0, 0.0], scaleQ=[sc, sc, 1.0], nq_A=2, nb_A=2)
+    V, G, Jac, sqRJ = P.jacobi_triatomic_grids(b)
+    psi = np.random.default_rng(L).standard_normal((b.nb_ba, nvec))
+    ref = o.hpsi(psi, V, None, type_op=10, G=G, Jac=Jac, sqRJ=sqRJ)
+    got = op.apply_host(psi)
+    assert rel(got, ref) < TOL

@@ -190,3 +190,71 @@ def test_type10_constant_G_matches_type1_spectrum():
     e10 = np.linalg.eigvalsh(0.5 * (H10 + H10.T))[:3]
     assert np.allclose(e1, [1, 2, 2], atol=1e-10)
     assert np.allclose(e10, [1, 2, 2], atol=1e-8)
+
+
+@pytest.mark.parametrize("nq", [1, 4, 11, 21])
+def test_legendre_basis_orthonormal(nq):
+    x, w, d0, d1, d2 = O.legendre_basis(nq, nq)
+    assert np.max(np.abs(d0.T @ (w[:, None] * d0) - np.eye(nq))) < 1e-13
+    assert np.all(np.diff(x) > 0) and x[0] > 0 and x[-1] < np.pi
+    # P_l(cos t)'' (t) = -l(l+1) P_l - cot(t) P_l'  (Legendre ODE in theta)
+    l = np.arange(nq)
+    lhs = d2 + (np.cos(x) / np.sin(x))[:, None] * d1
+    assert np.max(np.abs(lhs + l * (l + 1) * d0)) < 1e-9
+
+
+def test_type10_legendre_rigid_rotor_spectrum():
+    """-1/2 (1/sin t) d/dt sin t d/dt on the Legendre grid (type 10 with
+    G = 1, Jac = sin t, sqRJ = 1): eigenvalues l(l+1)/2, exactly for the
+    functions whose image stays in the basis (l <= nb - 2)."""
+    L = 8
+    s = O.SG4(1, L, kind=1)
+    nq = s.sum_nq
+    th = s.term_grid(0)[:, 0]
+    H = s.hpsi(np.eye(s.nb_ba), np.zeros(nq), None, type_op=10, G=np.ones(nq),
+               Jac=np.sin(th), sqRJ=np.ones(nq))
+    nb = s.nb_ba
+    l = np.arange(nb - 1)
+    # the operator is diagonal on P_0..P_{nb-2} with l(l+1)/2 on the diagonal
+    assert np.max(np.abs(np.diag(H)[: nb - 1] - l * (l + 1) / 2)) < 1e-9
+    off = H[: nb - 1, : nb - 1] - np.diag(np.diag(H)[: nb - 1])
+    assert np.max(np.abs(off)) < 1e-9
+
+
+def test_type10_jacobi_3d_ground_level_bound():
+    # the type-10 SG4 operator of the 3D Jacobi model is symmetric and its
+    # lowest level lies above the sum of the two Morse harmonic ZPEs minus
+    # anharmonic corrections (closed form for Morse: hw/2 - (hw)^2/(16 D))
+    import elvibrot_amd.potentials as P
+    s = O.SG4(3, 5, kind=[0, 0, 1], Q0=[3.0, 2.0, 0.0], nq_A=2, nb_A=2,
+              scaleQ=[np.sqrt(P.AMU * np.sqrt(0.4 / P.AMU))] * 2 + [1.0])
+    q = np.concatenate([s.term_grid(t) for t in range(s.nterms)])
+    R, r, th = q.T
+    V = P.morse(R, 0.2, 1.0, 3.0) + P.morse(r, 0.2, 1.0, 2.0) + 0.01 * np.cos(th) ** 2
+    G = np.zeros((len(R), 3, 3))
+    G[:, 0, 0] = G[:, 1, 1] = 1 / P.AMU
+    G[:, 2, 2] = 1 / (P.AMU * R ** 2) + 1 / (P.AMU * r ** 2)
+    H = s.hpsi(np.eye(s.nb_ba), V, None, type_op=10, G=G.reshape(-1), Jac=np.sin(th),
+               sqRJ=np.ones_like(th))
+    # d1 G d1 with pseudo-inverse derivative matrices is not exactly
+    # Hermitian on a finite grid (the reference measures this,
+    # sub_hermitic_H in sub_module_Davidson.f90:300-315); the lowest levels
+    # are real and well separated
+    w = np.sqrt(0.4 / P.AMU)
+    e_morse = 2 * (w / 2 - w ** 2 / (16 * 0.2))
+    ev = np.linalg.eigvals(H)
+    e0 = np.min(ev.real)
+    assert abs(ev[np.argmin(ev.real)].imag) < 1e-12
+    assert e_morse < e0 < e_morse + 0.0101
+    # converged in L: the same lowest level at L = 5 (above) and L = 8
+    s8 = O.SG4(3, 8, kind=[0, 0, 1], Q0=[3.0, 2.0, 0.0], nq_A=2, nb_A=2,
+               scaleQ=[np.sqrt(P.AMU * np.sqrt(0.4 / P.AMU))] * 2 + [1.0])
+    q8 = np.concatenate([s8.term_grid(t) for t in range(s8.nterms)])
+    R8, r8, t8 = q8.T
+    V8 = P.morse(R8, 0.2, 1.0, 3.0) + P.morse(r8, 0.2, 1.0, 2.0) + 0.01 * np.cos(t8) ** 2
+    G8 = np.zeros((len(R8), 3, 3))
+    G8[:, 0, 0] = G8[:, 1, 1] = 1 / P.AMU
+    G8[:, 2, 2] = 1 / (P.AMU * R8 ** 2) + 1 / (P.AMU * r8 ** 2)
+    H8 = s8.hpsi(np.eye(s8.nb_ba), V8, None, type_op=10, G=G8.reshape(-1), Jac=np.sin(t8),
+                 sqRJ=np.ones_like(t8))
+    assert abs(np.min(np.linalg.eigvals(H8).real) - e0) < 1e-7 * e0
