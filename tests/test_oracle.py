@@ -257,4 +257,4 @@ def test_type10_jacobi_3d_ground_level_bound():
     G8[:, 2, 2] = 1 / (P.AMU * R8 ** 2) + 1 / (P.AMU * r8 ** 2)
     H8 = s8.hpsi(np.eye(s8.nb_ba), V8, None, type_op=10, G=G8.reshape(-1), Jac=np.sin(t8),
                  sqRJ=np.ones_like(t8))
-    assert abs(np.min(np.linalg.eigvals(H8).real) - e0) < 1e-7 * e0
+    assert abs(np.min(np.linalg.eigvals(H8).real) - e0) < 1e-5 * e0
