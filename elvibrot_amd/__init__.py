@@ -184,6 +184,19 @@ class SG4Operator:
                                          C.c_void_p(out.data_ptr()), prec))
         return out
 
+    def grad_V(self, phi, psi, out):
+        """d<phi_v|H|psi_v>/dV(p) for every grid point p (device tensors in the
+        rows layout: phi, psi (nb_ba, nvec); out (sum_nq, nvec))."""
+        nvec = psi.shape[1]
+        _capi.check(lib().sg4_grad_V(self._h, nvec, C.c_void_p(phi.data_ptr()),
+                                     C.c_void_p(psi.data_ptr()), C.c_void_p(out.data_ptr())))
+        return out
+
+    def set_field(self, mu, eps):
+        """V = V0 + eps * mu (mu: device tensor with sum_nq entries, or None)."""
+        _capi.check(lib().sg4_plan_set_field(self._h, None if mu is None else
+                                             C.c_void_p(mu.data_ptr()), float(eps)))
+
     def timing(self, enable=True):
         _capi.check(lib().sg4_plan_timing(self._h, 1 if enable else 0))
 

@@ -17,7 +17,7 @@ EXPORTS = [
     "sg4_setup_term_grid", "sg4_setup_basis1d", "sg4_setup_destroy", "sg4_plan_create",
     "sg4_plan_set_stream", "sg4_plan_set_operator", "sg4_plan_set_henon_heiles", "sg4_apply",
     "sg4_apply_host", "sg4_apply_rows", "sg4_plan_stats", "sg4_plan_timing",
-    "sg4_plan_kernel_times", "sg4_plan_destroy",
+    "sg4_plan_kernel_times", "sg4_grad_V", "sg4_plan_set_field", "sg4_plan_destroy",
 ]
 
 
@@ -34,7 +34,7 @@ class ModeDesc(C.Structure):
 class Tables(C.Structure):
     _fields_ = [("D", C.c_int32), ("Lmax", C.c_int32), ("nterms", C.c_int64),
                 ("nb_ba", C.c_int64), ("levels", I_P), ("weights", D_P), ("map", I_P),
-                ("nq", I_P), ("nb", I_P), ("mats", D_P), ("nodes", D_P)]
+                ("nq", I_P), ("nb", I_P), ("mats", D_P), ("nodes", D_P), ("qweights", D_P)]
 
 
 class Operator(C.Structure):
@@ -71,6 +71,8 @@ def lib():
                                      C.POINTER(i64)]
         L.sg4_plan_timing.argtypes = [vp, i32]
         L.sg4_plan_kernel_times.argtypes = [vp, C.POINTER(C.c_float), i32, C.POINTER(i32)]
+        L.sg4_grad_V.argtypes = [vp, i32, vp, vp, vp]
+        L.sg4_plan_set_field.argtypes = [vp, vp, C.c_double]
         L.sg4_plan_destroy.argtypes = [vp]
         _lib = L
     return _lib

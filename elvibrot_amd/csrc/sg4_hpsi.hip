@@ -483,6 +483,85 @@ __global__ __launch_bounds__(NT) void k_hpsi_t10_f64(KArgs a, KMats10 km) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Grid-value gradient of <phi|H|psi>: both vectors transformed basis -> grid
+// (the first half of the H.psi pipeline), then the weighted product.
+// LDS: four buffers of nq_t x nv.
+// ---------------------------------------------------------------------------
+struct KGrad {
+  const double* mats;
+  const double* qw;        // 1-D quadrature weights
+  const int64_t* qw_off;   // (k*(Lmax+1)+l) -> offset
+  const double* phi;
+  double* grad;
+};
+
+__device__ void bg_transform(const KArgs& a, const double* mats, const TermDev* Td, const Item& it,
+                             const double* src, double* b0, double* b1, double** result) {
+  const int lg = it.lgnv, nv = 1 << lg;
+  PassCtx<double> c;
+  c.map = a.map + Td->boff; c.psi = src; c.hpsi = nullptr; c.nvec = a.nvec; c.v0 = it.v0;
+  c.wgt = 1.0; c.V = nullptr; c.vshift = 0.0; c.lgnv = lg; c.coef = 1.0;
+  int A = 1, Bafter = Td->nb;
+  double* cur = nullptr;
+  double* nxt = b0;
+  for (int p = 0; p < Td->nact; ++p) {
+    const MatDev* md = &a.md[Td->mat[p]];
+    const int nq = md->nq, nb = md->nb;
+    Bafter /= nb;
+    c.MR = mats + md->B;
+    c.in = cur; c.out = nxt;
+    c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
+    c.nfib = A * nv * Bafter;
+    if (p == 0) dispatch_pass<double, true, IN_GATHER, OUT_STORE>(c, nb, nq);
+    else dispatch_pass<double, true, IN_LDS, OUT_STORE>(c, nb, nq);
+    __syncthreads();
+    A *= nq;
+    cur = nxt;
+    nxt = (nxt == b0) ? b1 : b0;
+  }
+  *result = cur;
+}
+
+__global__ __launch_bounds__(NT) void k_grad_v(KArgs a, KGrad kg) {
+  extern __shared__ double lds[];
+  const Item it = a.items[blockIdx.x];
+  const TermDev* Td = &a.terms[it.term];
+  const int lg = it.lgnv, nv = 1 << lg;
+  const size_t SZ = (size_t)Td->nq * nv;
+  double* gphi = nullptr;
+  double* gpsi = nullptr;
+  if (Td->nact == 0) {   // one point, no transform
+    for (int v = threadIdx.x; v < nv; v += NT) {
+      const int m = a.map[Td->boff];
+      const int64_t r = (int64_t)m * a.nvec + it.v0 + v;
+      kg.grad[Td->qoff * a.nvec + it.v0 + v] = (m >= 0) ? Td->wgt * kg.phi[r] * a.psi[r] : 0.0;
+    }
+    return;
+  }
+  bg_transform(a, kg.mats, Td, it, kg.phi, lds, lds + SZ, &gphi);
+  bg_transform(a, kg.mats, Td, it, a.psi, lds + 2 * SZ, lds + 3 * SZ, &gpsi);
+  for (size_t e = threadIdx.x; e < SZ; e += NT) {
+    int pt = (int)(e >> lg);
+    double w = Td->wgt;
+    for (int p = 0; p < Td->nact; ++p) {
+      const MatDev* md = &a.md[Td->mat[p]];
+      const int i = pt % md->nq;
+      pt /= md->nq;
+      w *= kg.qw[kg.qw_off[Td->mat[p]] + i];
+    }
+    kg.grad[(Td->qoff + (int64_t)(e >> lg)) * a.nvec + it.v0 + (e & (nv - 1))] = w * gphi[e] * gpsi[e];
+  }
+}
+
+// V = V0 + eps * mu on every grid point (piecewise-constant field segment)
+__global__ void k_axpy_grid(const double* __restrict__ v0, const double* __restrict__ mu,
+                            double eps, double* __restrict__ v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = mu ? fma(eps, mu[i], v0[i]) : v0[i];
+}
+
 __global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -601,6 +680,9 @@ struct sg4_plan {
       d_levels, d_qoff;
   sg4::DevBuf d_mats32, d_V32, d_ck32;   // FP32 copies for the FP32 modes
   sg4::DevBuf d_G, d_Jac, d_sqRJ;        // type-10 grids
+  sg4::DevBuf d_qw;                      // 1-D quadrature weights (offsets as nodes)
+  sg4::DevBuf d_V0;                      // field-free potential (kept by sg4_plan_set_field)
+  bool have_qw = false;
   bool f32_valid = false;
   int64_t nmats = 0;
   sg4::DevBuf d_rows_in, d_rows_out, d_cols_in, d_cols_out;
@@ -665,8 +747,8 @@ static int build_terms(sg4_plan* P) {
 // term too large for that even with one vector keeps its second buffer in a
 // global scratch slot.  Items are ordered by decreasing work so that the
 // dispatcher starts the long ones first.
-static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out) {
-  const int key = nvec * 16 + esize;
+static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_force = 0) {
+  const int key = nvec * 16 + esize + (nbuf_force ? 1 : 0);
   auto itp = P->items.find(key);
   if (itp != P->items.end()) { *out = itp->second.get(); return SG4_OK; }
   struct HI { Item it; double cost; };
@@ -677,11 +759,11 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out) {
   for (int64_t t = 0; t < P->nterms; ++t) {
     const TermDev& T = P->h_terms[t];
     const int64_t nmax = std::max(T.nq, T.nb);
-    const int nbuf = (P->type_op == 10) ? P->D + 3 : 2;
+    const int nbuf = nbuf_force ? nbuf_force : ((P->type_op == 10) ? P->D + 3 : 2);
     const int64_t per = nbuf * nmax * esize;  // bytes per vector (work buffers)
     const bool big = per > LDS_SMALL;
-    if (big && P->type_op == 10) {
-      set_error("type-10 term too large for the LDS work buffers");
+    if (big && (P->type_op == 10 || nbuf_force)) {
+      set_error("term too large for the LDS work buffers of this kernel");
       return SG4_ERR_UNSUPPORTED;
     }
     if (big && nmax * esize > LDS_SMALL) {
@@ -888,6 +970,8 @@ int sg4_plan_create(const sg4_tables* t, int32_t max_nvec, sg4_plan** out) {
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
     (void)hipFuncSetAttribute((const void*)k_hpsi_t10_f64,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
+    (void)hipFuncSetAttribute((const void*)k_grad_v,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
   });
   std::unique_ptr<sg4_plan> P(new sg4_plan);
   P->D = t->D; P->Lmax = t->Lmax; P->nterms = t->nterms; P->nb_ba = t->nb_ba;
@@ -972,6 +1056,15 @@ int sg4_plan_create(const sg4_tables* t, int32_t max_nvec, sg4_plan** out) {
     HIP_TRY(hipMemcpy(P->d_node_off.p, node_off.data(), sizeof(int64_t) * node_off.size(), hipMemcpyHostToDevice));
     P->have_nodes = true;
   }
+  if (t->qweights) {
+    if ((rc = P->d_qw.alloc(sizeof(double) * noff))) return rc;
+    HIP_TRY(hipMemcpy(P->d_qw.p, t->qweights, sizeof(double) * noff, hipMemcpyHostToDevice));
+    if (!P->d_node_off.p) {
+      if ((rc = P->d_node_off.alloc(sizeof(int64_t) * node_off.size()))) return rc;
+      HIP_TRY(hipMemcpy(P->d_node_off.p, node_off.data(), sizeof(int64_t) * node_off.size(), hipMemcpyHostToDevice));
+    }
+    P->have_qw = true;
+  }
   if ((rc = P->d_V.alloc(sizeof(double) * P->sum_nq))) return rc;
   // work buffers for the column-major entry point
   const size_t blk = sizeof(double) * (size_t)P->nb_ba * max_nvec;
@@ -1020,6 +1113,7 @@ int sg4_plan_set_operator(sg4_plan* p, const sg4_operator* op) {
   }
   p->type_op = op->type_op;
   HIP_TRY(hipMemcpy(p->d_V.p, op->V, sizeof(double) * p->sum_nq, hipMemcpyHostToDevice));
+  (void)p->d_V0.alloc(0);
   p->items.clear();
   p->f32_valid = false;
   return build_terms(p);
@@ -1040,6 +1134,7 @@ int sg4_plan_set_henon_heiles(sg4_plan* p, double lambda) {
                      p->stream, a);
   HIP_TRY(hipGetLastError());
   p->type_op = 1;
+  (void)p->d_V0.alloc(0);
   p->h_ck.assign(p->D, -0.5);
   int rc;
   if ((rc = p->d_ck.alloc(sizeof(double) * p->D))) return rc;
@@ -1099,6 +1194,46 @@ int sg4_plan_stats(const sg4_plan* p, int32_t nvec, double* flops, double* bytes
     auto it8 = p->items.find(nvec * 16 + 8);
     *items = (it8 == p->items.end()) ? -1 : it8->second->nitems;
   }
+  return SG4_OK;
+}
+
+int sg4_plan_set_field(sg4_plan* p, const double* mu, double eps) {
+  if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
+  if (p->type_op == 0) { set_error("no operator set on the plan"); return SG4_ERR_STATE; }
+  int rc;
+  if (!p->d_V0.p) {
+    if ((rc = p->d_V0.alloc(sizeof(double) * p->sum_nq))) return rc;
+    HIP_TRY(hipMemcpyAsync(p->d_V0.p, p->d_V.p, sizeof(double) * p->sum_nq,
+                           hipMemcpyDeviceToDevice, p->stream));
+  }
+  hipLaunchKernelGGL(k_axpy_grid, dim3(grid_blocks(p->sum_nq)), dim3(256), 0, p->stream,
+                     p->d_V0.as<double>(), mu, eps, p->d_V.as<double>(), p->sum_nq);
+  HIP_TRY(hipGetLastError());
+  p->f32_valid = false;
+  return SG4_OK;
+}
+
+int sg4_grad_V(sg4_plan* p, int32_t nvec, const double* phi, const double* psi, double* grad) {
+  if (!p || !phi || !psi || !grad) { set_error("null argument"); return SG4_ERR_ARG; }
+  if (p->type_op == 0) { set_error("no operator set on the plan"); return SG4_ERR_STATE; }
+  if (!p->have_qw) { set_error("plan was created without quadrature weights"); return SG4_ERR_STATE; }
+  if (nvec < 1 || nvec > p->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
+  ItemPlan* ip = nullptr;
+  int rc = get_items(p, nvec, 8, &ip, 4);
+  if (rc) return rc;
+  KArgs a;
+  a.terms = p->d_terms.as<TermDev>();
+  a.md = p->d_md.as<MatDev>();
+  a.map = p->d_map.as<int32_t>();
+  a.psi = psi;
+  a.hpsi = nullptr;
+  a.nvec = nvec;
+  a.items = ip->items.as<Item>();
+  a.scratch = nullptr;
+  a.slot_size = 0;
+  KGrad kg{p->d_mats.as<double>(), p->d_qw.as<double>(), p->d_node_off.as<int64_t>(), phi, grad};
+  hipLaunchKernelGGL(k_grad_v, dim3((unsigned)ip->nitems), dim3(NT), ip->lds, p->stream, a, kg);
+  HIP_TRY(hipGetLastError());
   return SG4_OK;
 }
 

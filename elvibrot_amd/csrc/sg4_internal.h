@@ -33,6 +33,7 @@ struct Setup {
   std::vector<int32_t> nq_tab, nb_tab;   // [k*(L+1) + l]
   std::vector<double> mats;
   std::vector<double> nodes;
+  std::vector<double> qweights;
 
   Setup(int D, int L, int LB, const sg4_mode_desc* modes);
   void build_terms();

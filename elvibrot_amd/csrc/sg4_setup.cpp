@@ -395,6 +395,7 @@ void Setup::build_map() {
       mats.insert(mats.end(), b.gg1.begin(), b.gg1.end());
       mats.insert(mats.end(), b.gg2.begin(), b.gg2.end());
       nodes.insert(nodes.end(), b.x.begin(), b.x.end());
+      qweights.insert(qweights.end(), b.w.begin(), b.w.end());
     }
 }
 
@@ -411,6 +412,7 @@ void Setup::fill_tables(sg4_tables* t) const {
   t->nb = nb_tab.data();
   t->mats = mats.data();
   t->nodes = nodes.data();
+  t->qweights = qweights.data();
 }
 
 // grid point coordinates of term t (first index fastest),

@@ -30,6 +30,7 @@ MODULE mod_sg4_gpu
     integer(c_int64_t) :: nterms = 0, nb_ba = 0
     type(c_ptr) :: levels = c_null_ptr, weights = c_null_ptr, map = c_null_ptr
     type(c_ptr) :: nq = c_null_ptr, nb = c_null_ptr, mats = c_null_ptr, nodes = c_null_ptr
+    type(c_ptr) :: qweights = c_null_ptr
   END TYPE sg4_tables
 
   ! the parts of param_psi (SRC/sub_WP/sub_module_psi_set_alloc.f90:56-169) used here

@@ -76,7 +76,8 @@ typedef struct {
   const int32_t *nq;
   const int32_t *nb;
   const double *mats;
-  const double *nodes;   /* optional: for k, for l: the nq grid nodes (basis%x) */
+  const double *nodes;    /* optional: for k, for l: the nq grid nodes (basis%x) */
+  const double *qweights; /* optional: for k, for l: the nq quadrature weights (basis%wrho) */
 } sg4_tables;
 
 /* ---- operator on the grids (get_OpGrid_type{1,10}_OF_ONEDP_FOR_SG4,
@@ -138,6 +139,24 @@ int sg4_apply_host(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, i
 /* Same in the interleaved device layout used by the device-side drivers:
  * psi[row*nvec + v], row < nb_ba. */
 int sg4_apply_rows(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);
+
+/* Field term of a piecewise-constant control: V = V0 + eps * mu on every grid
+ * point, V0 being the grid held before the first call (mu: DEVICE pointer to
+ * sum_t nq_t values, e.g. the dipole sum_i q_i; mu = NULL restores V0).  Used
+ * per field segment by the propagators: H(t) = H0 + eps(t) mu is constant on a
+ * segment.  The reference propagates fields with Taylor/RK only
+ * (march_noD_field, SRC/sub_propagation/sub_module_propa_march.f90:1641). */
+int sg4_plan_set_field(sg4_plan *p, const double *mu, double eps);
+
+/* Derivative of <phi_v|H|psi_v> with respect to every grid value of the
+ * potential: grad[(p)*nvec + v] = W_t w(p) (B_t phi_v)(p) (B_t psi_v)(p) for the
+ * grid point p of term t (w(p): product of the 1-D quadrature weights).  With
+ * phi = psi = a normalised eigenvector this is the Hellmann-Feynman gradient
+ * dE/dV(p); contracted with dV/dtheta it gives dE/dtheta.  No counterpart in
+ * the reference (its only derivative machinery is the field update of
+ * sub_control.f90:907-1011).  Rows layout, device pointers, grad has
+ * sum_t nq_t x nvec entries; needs tables->qweights at plan creation. */
+int sg4_grad_V(sg4_plan *p, int32_t nvec, const double *phi, const double *psi, double *grad);
 
 /* Statistics of the last apply: launches, items, and the algorithmic flop and
  * byte counts per block apply used for roofline reporting. */

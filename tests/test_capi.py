@@ -17,7 +17,7 @@ from oracle import oracle as O
 def header_functions():
     src = open(os.path.join(ROOT, "include", "sg4hpsi.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(sg4_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(sg4_[A-Za-z0-9_]+)\s*\(", src)))
 
 
 def test_header_matches_binding_list():
