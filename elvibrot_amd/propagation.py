@@ -166,22 +166,23 @@ def spectral_bounds(op, nsteps=60, seed=1, device="cuda:0", margin=0.05):
     return lo - margin * d, hi + margin * d
 
 
-def propagate_field(prop, psi0, eps, steps_per_segment, store_every=None):
-    """Forward propagation through the segments; returns the final state and,
-    if store_every is set, the checkpoints {segment index: state at its start}
-    and the number of polynomials used per segment."""
+def propagate_field(prop, psi0, eps, steps_per_segment, store_every=None, npolys=None):
+    """Forward propagation through the segments; returns the final state, the
+    checkpoints {segment index: state at its start} (if store_every is set)
+    and the number of polynomials used per step (or the given ones)."""
     psi = psi0
     cps = {}
-    npolys = []
+    used = []
     for s, e in enumerate(eps):
         if store_every and s % store_every == 0:
             cps[s] = psi
         prop.set_field(e)
         nps = []
-        for _ in range(steps_per_segment):
-            psi, n = prop.step(psi)
+        for j in range(steps_per_segment):
+            psi, n = prop.step(psi, npoly=None if npolys is None else npolys[s][j])
             nps.append(n)
-        npolys.append(nps)
+        used.append(nps)
+    npolys = used
     prop.set_field(0.0)
     return psi, cps, npolys
 

@@ -63,9 +63,10 @@ def test_field_gradient_vs_finite_differences(cuda):
     target[0, 1] = np.sqrt(0.5)
     target[0, 0] = np.sqrt(0.5)
     J, g = overlap_gradient(prop, psi0, target, eps, steps, checkpoint_every=4)
+    _, _, npolys = propagate_field(prop, psi0, eps, steps)
 
-    def Jof(e):
-        psiT, _, _ = propagate_field(prop, psi0, e, steps)
+    def Jof(e):   # the same discrete map: polynomial orders of the forward run
+        psiT, _, _ = propagate_field(prop, psi0, e, steps, npolys=npolys)
         return abs(torch.sum(target.conj() * psiT).item()) ** 2
 
     assert abs(Jof(eps) - J) < 1e-13
