@@ -197,6 +197,24 @@ class SG4Operator:
         _capi.check(lib().sg4_plan_set_field(self._h, None if mu is None else
                                              C.c_void_p(mu.data_ptr()), float(eps)))
 
+    def autotune(self, nvec, prec=PREC_FP64):
+        """Pick the fastest launch parameters for blocks of nvec vectors; returns ms."""
+        ms = C.c_float()
+        _capi.check(lib().sg4_plan_autotune(self._h, nvec, prec, C.byref(ms)))
+        return ms.value
+
+    def config(self, nvec, prec=PREC_FP64):
+        c = (C.c_int32 * 4)()
+        _capi.check(lib().sg4_plan_config(self._h, nvec, prec, c))
+        return dict(nthreads=c[0], lds_budget=c[1], grid=c[2], xcd=c[3])
+
+    def apply_graph(self, psi, out, prec=PREC_FP64):
+        """apply_rows through a captured HIP graph (rows layout tensors)."""
+        nvec = psi.shape[1]
+        _capi.check(lib().sg4_apply_graph(self._h, nvec, C.c_void_p(psi.data_ptr()),
+                                          C.c_void_p(out.data_ptr()), prec))
+        return out
+
     def timing(self, enable=True):
         _capi.check(lib().sg4_plan_timing(self._h, 1 if enable else 0))
 

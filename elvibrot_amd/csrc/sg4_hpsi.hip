@@ -44,7 +44,7 @@ void set_error(const std::string& msg) { g_last_error = msg; }
   } while (0)
 
 constexpr int MAXACT = 16;
-constexpr int NT = 256;                 // threads per workgroup
+constexpr int NT = 512;                 // maximum threads per workgroup (launch bound)
 constexpr int LDS_SMALL = 64 * 1024;    // LDS budget of the common class (2 WG/CU)
 constexpr int LDS_MAX = 160 * 1024;     // gfx950 LDS per workgroup
 
@@ -83,6 +83,7 @@ struct KArgs {
   double* scratch;
   int64_t slot_size;
   int nvec;
+  int nitems;
 };
 
 // ---------------------------------------------------------------------------
@@ -165,7 +166,7 @@ __device__ __forceinline__ void emit(const PassCtx<T>& c, int av, int b, int i, 
 // vector registers across the fibre loop.
 template <class T, int NI, int NO, int KIN, int KOUT>
 __device__ __forceinline__ void mode_pass(const PassCtx<T>& c) {
-  for (int f = threadIdx.x; f < c.nfib; f += NT) {
+  for (int f = threadIdx.x; f < c.nfib; f += blockDim.x) {
     const int b = fdiv(f, c.AV, c.invAV);
     const int av = f - b * c.AV;
     T x[NI];
@@ -199,7 +200,7 @@ __device__ __forceinline__ void mode_pass(const PassCtx<T>& c) {
 // held in registers, which keeps this path's register use small.
 template <class T, int KIN, int KOUT>
 __device__ __noinline__ void mode_pass_generic(const PassCtx<T>& c, int NI, int NO) {
-  for (int f = threadIdx.x; f < c.nfib; f += NT) {
+  for (int f = threadIdx.x; f < c.nfib; f += blockDim.x) {
     const int b = fdiv(f, c.AV, c.invAV);
     const int av = f - b * c.AV;
     for (int i = 0; i < NO; ++i) {
@@ -340,19 +341,29 @@ __device__ __forceinline__ void term_t1(const KArgs& a, const KMats<T>& km, cons
   }
 }
 
+// The grid is either one workgroup per item or a persistent grid that walks
+// the (cost-sorted) item list with stride gridDim.x.
 __global__ __launch_bounds__(NT) void k_hpsi_t1_f64(KArgs a, KMats<double> km) {
   extern __shared__ double lds[];
-  const Item it = a.items[blockIdx.x];
-  if (it.big) term_t1<double, true>(a, km, it, lds);
-  else term_t1<double, false>(a, km, it, lds);
+  for (int w = blockIdx.x; w < a.nitems; w += gridDim.x) {
+    if (w != (int)blockIdx.x) __syncthreads();   // LDS reuse between items
+    const Item it = a.items[w];
+    if (it.term < 0) continue;
+    if (it.big) term_t1<double, true>(a, km, it, lds);
+    else term_t1<double, false>(a, km, it, lds);
+  }
 }
 
 __global__ __launch_bounds__(NT) void k_hpsi_t1_f32(KArgs a, KMats<float> km) {
   extern __shared__ double lds_d[];
   float* lds = (float*)lds_d;
-  const Item it = a.items[blockIdx.x];
-  if (it.big) term_t1<float, true>(a, km, it, lds);
-  else term_t1<float, false>(a, km, it, lds);
+  for (int w = blockIdx.x; w < a.nitems; w += gridDim.x) {
+    if (w != (int)blockIdx.x) __syncthreads();
+    const Item it = a.items[w];
+    if (it.term < 0) continue;
+    if (it.big) term_t1<float, true>(a, km, it, lds);
+    else term_t1<float, false>(a, km, it, lds);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -411,7 +422,7 @@ __global__ __launch_bounds__(NT) void k_hpsi_t10_f64(KArgs a, KMats10 km) {
   }
   double* pg = cur;   // psi on the grid
   double* S = nxt;    // scratch: u, then w_i
-  for (size_t e = threadIdx.x; e < SZ; e += NT) S[e] = km.sqRJ[q0 + (e >> lg)] * pg[e];
+  for (size_t e = threadIdx.x; e < SZ; e += blockDim.x) S[e] = km.sqRJ[q0 + (e >> lg)] * pg[e];
   __syncthreads();
   // ---- u_j = d1_j u ----
   {
@@ -433,7 +444,7 @@ __global__ __launch_bounds__(NT) void k_hpsi_t10_f64(KArgs a, KMats10 km) {
   {
     int Ab = 1, Aa = Td->nq;
     for (int i = 0; i < D; ++i) {
-      for (size_t e = threadIdx.x; e < SZ; e += NT) {
+      for (size_t e = threadIdx.x; e < SZ; e += blockDim.x) {
         const int64_t q = q0 + (int64_t)(e >> lg);
         const double* Gq = km.G + (int64_t)D * D * q;
         double w = 0.0;
@@ -454,7 +465,7 @@ __global__ __launch_bounds__(NT) void k_hpsi_t10_f64(KArgs a, KMats10 km) {
       Ab *= nq;
     }
   }
-  for (size_t e = threadIdx.x; e < SZ; e += NT) {
+  for (size_t e = threadIdx.x; e < SZ; e += blockDim.x) {
     const int64_t q = q0 + (int64_t)(e >> lg);
     O[e] = -0.5 * O[e] / (km.Jac[q] * km.sqRJ[q]) + km.V[q] * pg[e];
   }
@@ -532,7 +543,7 @@ __global__ __launch_bounds__(NT) void k_grad_v(KArgs a, KGrad kg) {
   double* gphi = nullptr;
   double* gpsi = nullptr;
   if (Td->nact == 0) {   // one point, no transform
-    for (int v = threadIdx.x; v < nv; v += NT) {
+    for (int v = threadIdx.x; v < nv; v += blockDim.x) {
       const int m = a.map[Td->boff];
       const int64_t r = (int64_t)m * a.nvec + it.v0 + v;
       kg.grad[Td->qoff * a.nvec + it.v0 + v] = (m >= 0) ? Td->wgt * kg.phi[r] * a.psi[r] : 0.0;
@@ -541,7 +552,7 @@ __global__ __launch_bounds__(NT) void k_grad_v(KArgs a, KGrad kg) {
   }
   bg_transform(a, kg.mats, Td, it, kg.phi, lds, lds + SZ, &gphi);
   bg_transform(a, kg.mats, Td, it, a.psi, lds + 2 * SZ, lds + 3 * SZ, &gpsi);
-  for (size_t e = threadIdx.x; e < SZ; e += NT) {
+  for (size_t e = threadIdx.x; e < SZ; e += blockDim.x) {
     int pt = (int)(e >> lg);
     double w = Td->wgt;
     for (int p = 0; p < Td->nact; ++p) {
@@ -661,6 +672,14 @@ struct ItemPlan {
   int lds = 0;
 };
 
+// Launch configuration of the H.psi kernel; chosen by sg4_plan_autotune.
+struct LaunchCfg {
+  int nthreads = 256;          // 256 or 512
+  int lds_budget = LDS_SMALL;  // bytes of work buffers per workgroup
+  int grid = 0;                // 0: one workgroup per item, else persistent grid
+  int xcd = 0;                 // 1: keep the items of a term on one XCD
+};
+
 }  // namespace sg4
 
 struct sg4_plan {
@@ -686,7 +705,11 @@ struct sg4_plan {
   bool f32_valid = false;
   int64_t nmats = 0;
   sg4::DevBuf d_rows_in, d_rows_out, d_cols_in, d_cols_out;
-  std::map<int, std::unique_ptr<sg4::ItemPlan>> items;  // per nvec
+  std::map<int64_t, std::unique_ptr<sg4::ItemPlan>> items;  // per (nvec, size, budget, order)
+  std::map<int, sg4::LaunchCfg> tuned;                       // per (nvec, precision)
+  hipStream_t own_stream = nullptr;                          // for graph capture on the null stream
+  struct GraphEntry { int nvec, prec; const double* psi; double* hpsi; hipGraphExec_t exec; };
+  std::vector<GraphEntry> graphs;
   double flops_per_vec = 0, bytes_fixed = 0;
   // optional kernel timing: HIP events around the H.psi launch of each apply
   bool timing = false;
@@ -695,6 +718,8 @@ struct sg4_plan {
   size_t ev_used = 0;
   ~sg4_plan() {
     for (auto& e : ev_pairs) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
   }
 };
 
@@ -743,12 +768,17 @@ static int build_terms(sg4_plan* P) {
 }
 
 // Work items for a block of nvec vectors: for every term the vectors are cut
-// into power-of-two chunks so that both work buffers fit the LDS budget; a
+// into power-of-two chunks so that the work buffers fit the LDS budget; a
 // term too large for that even with one vector keeps its second buffer in a
-// global scratch slot.  Items are ordered by decreasing work so that the
-// dispatcher starts the long ones first.
-static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_force = 0) {
-  const int key = nvec * 16 + esize + (nbuf_force ? 1 : 0);
+// global scratch slot.  Items are ordered by decreasing work so that the long
+// ones start first.  With xcd = 1 the items of one term are placed at
+// positions congruent modulo 8 (workgroups are dealt round-robin to the eight
+// XCDs), so the chunks of a term share one L2 for their gathers; terms are
+// given to the XCD with the least accumulated work.
+static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_force = 0,
+                     int budget = LDS_SMALL, int xcd = 0) {
+  const int64_t key = ((((int64_t)nvec * 16 + esize + (nbuf_force ? 1 : 0)) * 1024 +
+                        budget / 1024) << 1) | (xcd ? 1 : 0);
   auto itp = P->items.find(key);
   if (itp != P->items.end()) { *out = itp->second.get(); return SG4_OK; }
   struct HI { Item it; double cost; };
@@ -761,18 +791,18 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
     const int64_t nmax = std::max(T.nq, T.nb);
     const int nbuf = nbuf_force ? nbuf_force : ((P->type_op == 10) ? P->D + 3 : 2);
     const int64_t per = nbuf * nmax * esize;  // bytes per vector (work buffers)
-    const bool big = per > LDS_SMALL;
+    const bool big = per > budget;
     if (big && (P->type_op == 10 || nbuf_force)) {
       set_error("term too large for the LDS work buffers of this kernel");
       return SG4_ERR_UNSUPPORTED;
     }
-    if (big && nmax * esize > LDS_SMALL) {
-      set_error("term with " + std::to_string(nmax) + " grid points exceeds the LDS capacity");
+    if (big && nmax * esize > budget) {
+      set_error("term with " + std::to_string(nmax) + " grid points exceeds the LDS budget");
       return SG4_ERR_UNSUPPORTED;
     }
     int gmax = 0;
     if (!big)
-      while ((1 << (gmax + 1)) <= nvec && per * (1 << (gmax + 1)) <= LDS_SMALL) ++gmax;
+      while ((1 << (gmax + 1)) <= nvec && per * (1 << (gmax + 1)) <= budget) ++gmax;
     double work = 0;
     for (int p = 0; p < T.nact; ++p) work += P->h_md[T.mat[p]].nq;
     work = (work + 1.0) * (double)nmax;
@@ -794,9 +824,34 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
     }
   }
   std::stable_sort(all.begin(), all.end(), [](const HI& x, const HI& y) { return x.cost > y.cost; });
+  std::vector<Item> h;
+  h.reserve(all.size());
+  if (!xcd) {
+    for (auto& x : all) h.push_back(x.it);
+  } else {
+    std::vector<std::vector<Item>> lane(8);
+    double acc[8] = {0};
+    std::map<int32_t, int> owner;
+    for (auto& x : all) {
+      auto o = owner.find(x.it.term);
+      int l;
+      if (o == owner.end()) {
+        l = (int)(std::min_element(acc, acc + 8) - acc);
+        owner[x.it.term] = l;
+      } else {
+        l = o->second;
+      }
+      lane[l].push_back(x.it);
+      acc[l] += x.cost;
+    }
+    size_t m = 0;
+    for (auto& v : lane) m = std::max(m, v.size());
+    Item pad{};
+    pad.term = -1;
+    for (size_t i = 0; i < m; ++i)
+      for (int l = 0; l < 8; ++l) h.push_back(i < lane[l].size() ? lane[l][i] : pad);
+  }
   auto ip = std::make_unique<ItemPlan>();
-  std::vector<Item> h(all.size());
-  for (size_t i = 0; i < all.size(); ++i) h[i] = all[i].it;
   ip->nitems = (int64_t)h.size();
   ip->nbig = nbig;
   ip->slot_size = slot;
@@ -832,7 +887,7 @@ static int ensure_f32(sg4_plan* P) {
   return SG4_OK;
 }
 
-static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, int prec) {
+static int check_apply(sg4_plan* P, int nvec, int prec) {
   if (P->type_op == 0) { set_error("no operator set on the plan"); return SG4_ERR_STATE; }
   if (nvec < 1 || nvec > P->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
   if (prec != SG4_PREC_FP64 && prec != SG4_PREC_F32) {
@@ -843,11 +898,32 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
     set_error("the type-10 operator is available in FP64 only");
     return SG4_ERR_UNSUPPORTED;
   }
-  const int esize = (prec == SG4_PREC_FP64) ? 8 : 4;
+  return SG4_OK;
+}
+
+static LaunchCfg cfg_for(sg4_plan* P, int nvec, int prec) {
+  auto it = P->tuned.find(nvec * 16 + prec);
+  return it == P->tuned.end() ? LaunchCfg{} : it->second;
+}
+
+// Everything an apply needs that may allocate or copy (FP32 copies, item
+// lists); done before any stream capture.
+static int prepare(sg4_plan* P, int nvec, int prec, const LaunchCfg& cfg, ItemPlan** ip) {
   int rc;
+  const int esize = (prec == SG4_PREC_FP64) ? 8 : 4;
   if (esize == 4 && (rc = ensure_f32(P))) return rc;
+  const bool t10 = P->type_op == 10;
+  return get_items(P, nvec, esize, ip, 0, t10 ? LDS_SMALL : cfg.lds_budget, t10 ? 0 : cfg.xcd);
+}
+
+static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, int prec,
+                      const LaunchCfg* cfgp = nullptr) {
+  int rc;
+  if ((rc = check_apply(P, nvec, prec))) return rc;
+  const LaunchCfg cfg = cfgp ? *cfgp : cfg_for(P, nvec, prec);
   ItemPlan* ip = nullptr;
-  if ((rc = get_items(P, nvec, esize, &ip))) return rc;
+  if ((rc = prepare(P, nvec, prec, cfg, &ip))) return rc;
+  const int esize = (prec == SG4_PREC_FP64) ? 8 : 4;
   const int64_t n = P->nb_ba * nvec;
   hipLaunchKernelGGL(k_fill, dim3(grid_blocks(n)), dim3(256), 0, P->stream, hpsi, n, 0.0);
   KArgs a;
@@ -857,6 +933,7 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
   a.psi = psi;
   a.hpsi = hpsi;
   a.nvec = nvec;
+  a.nitems = (int)ip->nitems;
   a.items = ip->items.as<Item>();
   a.scratch = ip->scratch.as<double>();
   a.slot_size = ip->slot_size;
@@ -872,16 +949,18 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
     ++P->ev_used;
     HIP_TRY(hipEventRecord(P->ev0, P->stream));
   }
+  const unsigned grid = (unsigned)((cfg.grid > 0 && cfg.grid < ip->nitems) ? cfg.grid : ip->nitems);
+  const dim3 blk(P->type_op == 10 ? 256 : cfg.nthreads);
   if (P->type_op == 10) {
     KMats10 km{P->d_mats.as<double>(), P->d_V.as<double>(), P->d_G.as<double>(),
                P->d_Jac.as<double>(), P->d_sqRJ.as<double>()};
-    hipLaunchKernelGGL(k_hpsi_t10_f64, dim3((unsigned)ip->nitems), dim3(NT), ip->lds, P->stream, a, km);
+    hipLaunchKernelGGL(k_hpsi_t10_f64, dim3((unsigned)ip->nitems), blk, ip->lds, P->stream, a, km);
   } else if (esize == 8) {
     KMats<double> km{P->d_mats.as<double>(), P->d_V.as<double>(), P->d_ck.as<double>()};
-    hipLaunchKernelGGL(k_hpsi_t1_f64, dim3((unsigned)ip->nitems), dim3(NT), ip->lds, P->stream, a, km);
+    hipLaunchKernelGGL(k_hpsi_t1_f64, dim3(grid), blk, ip->lds, P->stream, a, km);
   } else {
     KMats<float> km{P->d_mats32.as<float>(), P->d_V32.as<float>(), P->d_ck32.as<float>()};
-    hipLaunchKernelGGL(k_hpsi_t1_f32, dim3((unsigned)ip->nitems), dim3(NT), ip->lds, P->stream, a, km);
+    hipLaunchKernelGGL(k_hpsi_t1_f32, dim3(grid), blk, ip->lds, P->stream, a, km);
   }
   if (P->timing) HIP_TRY(hipEventRecord(P->ev1, P->stream));
   HIP_TRY(hipGetLastError());
@@ -1229,11 +1308,117 @@ int sg4_grad_V(sg4_plan* p, int32_t nvec, const double* phi, const double* psi, 
   a.hpsi = nullptr;
   a.nvec = nvec;
   a.items = ip->items.as<Item>();
+  a.nitems = (int)ip->nitems;
   a.scratch = nullptr;
   a.slot_size = 0;
   KGrad kg{p->d_mats.as<double>(), p->d_qw.as<double>(), p->d_node_off.as<int64_t>(), phi, grad};
   hipLaunchKernelGGL(k_grad_v, dim3((unsigned)ip->nitems), dim3(NT), ip->lds, p->stream, a, kg);
   HIP_TRY(hipGetLastError());
+  return SG4_OK;
+}
+
+int sg4_plan_autotune(sg4_plan* p, int32_t nvec, int32_t prec, float* best_ms) {
+  if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
+  int rc;
+  if ((rc = check_apply(p, nvec, prec))) return rc;
+  if (p->type_op == 10) { if (best_ms) *best_ms = 0.0f; return SG4_OK; }
+  DevBuf x, y;
+  const size_t bytes = sizeof(double) * (size_t)p->nb_ba * nvec;
+  if ((rc = x.alloc(bytes)) || (rc = y.alloc(bytes))) return rc;
+  hipLaunchKernelGGL(k_fill, dim3(grid_blocks(p->nb_ba * nvec)), dim3(256), 0, p->stream,
+                     x.as<double>(), p->nb_ba * nvec, 1.0);
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const bool timing = p->timing;
+  p->timing = false;
+  LaunchCfg best;
+  float bms = 1e30f;
+  int cus = 256;
+  {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cus = prop.multiProcessorCount;
+  }
+  const int threads[] = {256, 512};
+  const int budgets[] = {32 * 1024, 48 * 1024, 64 * 1024, 80 * 1024};
+  const int grids[] = {0, 4, 8};   // 0: one per item; k: k workgroups per CU, persistent
+  for (int nt : threads)
+    for (int bu : budgets)
+      for (int gk : grids)
+        for (int xc = 0; xc < 2; ++xc) {
+          LaunchCfg c;
+          c.nthreads = nt; c.lds_budget = bu; c.grid = gk ? gk * cus : 0; c.xcd = xc;
+          ItemPlan* ip = nullptr;
+          if (prepare(p, nvec, prec, c, &ip) != SG4_OK) { (void)hipGetLastError(); continue; }
+          if ((rc = apply_rows(p, nvec, x.as<double>(), y.as<double>(), prec, &c))) break;
+          HIP_TRY(hipEventRecord(e0, p->stream));
+          for (int r = 0; r < 3; ++r)
+            if ((rc = apply_rows(p, nvec, x.as<double>(), y.as<double>(), prec, &c))) break;
+          HIP_TRY(hipEventRecord(e1, p->stream));
+          HIP_TRY(hipEventSynchronize(e1));
+          float ms = 0;
+          HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+          ms /= 3;
+          if (ms < bms) { bms = ms; best = c; }
+        }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  p->timing = timing;
+  if (rc) return rc;
+  p->tuned[nvec * 16 + prec] = best;
+  if (best_ms) *best_ms = bms;
+  return SG4_OK;
+}
+
+int sg4_plan_config(const sg4_plan* p, int32_t nvec, int32_t prec, int32_t* cfg4) {
+  if (!p || !cfg4) { set_error("null argument"); return SG4_ERR_ARG; }
+  auto it = p->tuned.find(nvec * 16 + prec);
+  const LaunchCfg c = it == p->tuned.end() ? LaunchCfg{} : it->second;
+  cfg4[0] = c.nthreads; cfg4[1] = c.lds_budget; cfg4[2] = c.grid; cfg4[3] = c.xcd;
+  return SG4_OK;
+}
+
+int sg4_apply_graph(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_t prec) {
+  if (!p || !psi || !hpsi) { set_error("null argument"); return SG4_ERR_ARG; }
+  int rc;
+  if ((rc = check_apply(p, nvec, prec))) return rc;
+  hipStream_t user = p->stream;
+  if (!p->own_stream) HIP_TRY(hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking));
+  sg4_plan::GraphEntry* g = nullptr;
+  for (auto& e : p->graphs)
+    if (e.nvec == nvec && e.prec == prec && e.psi == psi && e.hpsi == hpsi) g = &e;
+  if (!g) {
+    ItemPlan* ip = nullptr;
+    if ((rc = prepare(p, nvec, prec, cfg_for(p, nvec, prec), &ip))) return rc;
+    HIP_TRY(hipStreamSynchronize(user));
+    const bool timing = p->timing;
+    p->timing = false;
+    p->stream = p->own_stream;
+    hipGraph_t graph;
+    HIP_TRY(hipStreamBeginCapture(p->own_stream, hipStreamCaptureModeThreadLocal));
+    rc = apply_rows(p, nvec, psi, hpsi, prec);
+    hipError_t e = hipStreamEndCapture(p->own_stream, &graph);
+    p->stream = user;
+    p->timing = timing;
+    if (rc) return rc;
+    if (e != hipSuccess) { set_error("stream capture failed"); return SG4_ERR_HIP; }
+    hipGraphExec_t exec;
+    HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(graph);
+    p->graphs.push_back({nvec, prec, psi, hpsi, exec});
+    g = &p->graphs.back();
+  }
+  // order the replay after the caller's stream and the caller's stream after it
+  hipEvent_t ev;
+  HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(ev, user));
+  HIP_TRY(hipStreamWaitEvent(p->own_stream, ev, 0));
+  HIP_TRY(hipGraphLaunch(g->exec, p->own_stream));
+  HIP_TRY(hipEventRecord(ev, p->own_stream));
+  HIP_TRY(hipStreamWaitEvent(user, ev, 0));
+  (void)hipEventDestroy(ev);
   return SG4_OK;
 }
 

@@ -158,6 +158,20 @@ int sg4_plan_set_field(sg4_plan *p, const double *mu, double eps);
  * sum_t nq_t x nvec entries; needs tables->qweights at plan creation. */
 int sg4_grad_V(sg4_plan *p, int32_t nvec, const double *phi, const double *psi, double *grad);
 
+/* Autotuner: times the H.psi kernel for blocks of nvec vectors over its
+ * launch parameters (256/512 threads, LDS budget per workgroup, one workgroup
+ * per work item or a persistent grid, XCD-grouped item order) and keeps the
+ * fastest for later applies; best_ms receives its time.  Allocates two
+ * temporary blocks.  The reference's only tuner scans OpenMP thread counts
+ * (Tune_SG4threads_HPsi, SRC/sub_main/vib.f90:2013-2123). */
+int sg4_plan_autotune(sg4_plan *p, int32_t nvec, int32_t prec, float *best_ms);
+/* the launch parameters in use for (nvec, prec): threads, LDS budget, grid, xcd */
+int sg4_plan_config(const sg4_plan *p, int32_t nvec, int32_t prec, int32_t *cfg4);
+/* sg4_apply_rows through a captured HIP graph (zeroing + H.psi kernel), one
+ * graph per (nvec, prec, psi, hpsi); replays are ordered after the plan's
+ * stream and the plan's stream waits for them. */
+int sg4_apply_graph(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);
+
 /* Statistics of the last apply: launches, items, and the algorithmic flop and
  * byte counts per block apply used for roofline reporting. */
 int sg4_plan_stats(const sg4_plan *p, int32_t nvec, double *flops, double *bytes,

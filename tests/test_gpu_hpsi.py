@@ -148,3 +148,38 @@ def test_type10_jacobi_parity(cuda, L, nvec):
     ref = o.hpsi(psi, V, None, type_op=10, G=G, Jac=Jac, sqRJ=sqRJ)
     got = op.apply_host(psi)
     assert rel(got, ref) < TOL
+
+
+def test_autotuned_config_keeps_parity(cuda):
+    import torch
+    D, L, nvec = 6, 6, 16
+    b, op, o, V = hh_pair(D, L)
+    ms = op.autotune(nvec)
+    cfg = op.config(nvec)
+    print("autotune", ms, cfg)
+    assert ms > 0 and cfg["nthreads"] in (256, 512)
+    psi = np.random.default_rng(4).standard_normal((b.nb_ba, nvec))
+    ref = o.hpsi(psi, V, -0.5 * np.ones(D))
+    assert rel(op.apply_host(psi), ref) < TOL
+
+
+def test_graph_replays_match_eager(cuda):
+    """Captured graph (fill kernel + H.psi kernel) replayed several times with
+    eager work in between; every replay equals the eager result."""
+    import torch
+    D, L, nvec = 6, 5, 8
+    b, op, o, V = hh_pair(D, L)
+    g = torch.Generator(device=cuda).manual_seed(3)
+    X = torch.randn(b.nb_ba, nvec, device=cuda, dtype=torch.float64, generator=g)
+    Y = torch.empty_like(X)
+    ref = torch.empty_like(X)
+    op.apply_rows(X, ref)
+    for r in range(4):
+        Y.fill_(float("nan"))
+        op.apply_graph(X, Y)
+        torch.cuda.synchronize()
+        assert torch.max(torch.abs(Y - ref)).item() <= 1e-13 * torch.max(torch.abs(ref)).item()
+        Z = torch.empty_like(X)
+        op.apply_rows(X * (r + 1), Z)          # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.max(torch.abs(Z - (r + 1) * ref)).item() <= 1e-12 * torch.max(torch.abs(Z)).item()
