@@ -23,7 +23,7 @@ import numpy as np
 from . import _capi
 from ._capi import SG4Error, lib, PREC_FP64, PREC_F32, PREC_F32X2, BASIS_HO, BASIS_LEGENDRE
 
-__all__ = ["SG4Basis", "SG4Operator", "SG4Error", "sub_TabOpPsi_FOR_SGtype4", "lib",
+__all__ = ["SG4Basis", "SG4Operator", "SG4Batch", "SG4Error", "sub_TabOpPsi_FOR_SGtype4", "lib",
            "PREC_FP64", "PREC_F32", "PREC_F32X2", "BASIS_HO", "BASIS_LEGENDRE",
            "henon_heiles_basis"]
 
@@ -228,6 +228,61 @@ class SG4Operator:
         f, b, n = C.c_double(), C.c_double(), C.c_int64()
         _capi.check(lib().sg4_plan_stats(self._h, nvec, C.byref(f), C.byref(b), C.byref(n)))
         return dict(flops=f.value, bytes=b.value, items=n.value)
+
+
+class SG4Batch(SG4Operator):
+    """Several independent SG4 operators in one plan (block-diagonal over the
+    concatenated packed rows; one launch per apply for all instances).
+
+    potential: ("henon_heiles", lam) for all instances, or a list of V grids
+    (one per instance) with a list of ck arrays."""
+
+    def __init__(self, bases, max_nvec=16, potential=("henon_heiles", 0.111803), ck=None,
+                 stream=None):
+        self.bases = list(bases)
+        self.max_nvec = max_nvec
+        self.type_op = 1
+        n = len(self.bases)
+        arr = (_capi.Tables * n)()
+        for i, b in enumerate(self.bases):
+            arr[i] = b.tables
+        h = C.c_void_p()
+        _capi.check(lib().sg4_plan_create_batch(n, arr, max_nvec, C.byref(h)))
+        self._h = h
+        self.row0 = np.cumsum([0] + [b.nb_ba for b in self.bases])
+        self.q0 = np.cumsum([0] + [b.sum_nq for b in self.bases])
+        self.basis = _BatchShape(self.bases, self.row0, self.q0)
+        if stream is not None:
+            _capi.check(lib().sg4_plan_set_stream(h, C.c_void_p(stream)))
+        if isinstance(potential, tuple) and potential[0] == "henon_heiles":
+            _capi.check(lib().sg4_plan_set_henon_heiles(h, float(potential[1])))
+            return
+        V = np.ascontiguousarray(np.concatenate(potential), dtype=np.float64)
+        ckv = np.ascontiguousarray(np.concatenate(
+            [(-0.5 * np.ones(b.D)) for b in self.bases] if ck is None else ck), dtype=np.float64)
+        op = _capi.Operator()
+        op.type_op = 1
+        op.V = V.ctypes.data_as(_capi.D_P)
+        op.ck = ckv.ctypes.data_as(_capi.D_P)
+        _capi.check(lib().sg4_plan_set_operator(h, C.byref(op)))
+
+
+class _BatchShape:
+    """What the drivers read from ``op.basis`` for a batch."""
+
+    def __init__(self, bases, row0, q0):
+        self.bases = bases
+        self.nb_ba = int(row0[-1])
+        self.sum_nq = int(q0[-1])
+        self.D = sum(b.D for b in bases)
+        self.nterms = sum(b.nterms for b in bases)
+
+    def term_grid(self, t):
+        for b in self.bases:
+            if t < b.nterms:
+                return b.term_grid(t)
+            t -= b.nterms
+        raise IndexError(t)
 
 
 def sub_TabOpPsi_FOR_SGtype4(Psi, OpPsi, para_Op, prec=PREC_FP64):

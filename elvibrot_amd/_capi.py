@@ -15,6 +15,7 @@ BASIS_HO, BASIS_LEGENDRE = 0, 1
 EXPORTS = [
     "sg4_last_error", "sg4_setup_create", "sg4_setup_sizes", "sg4_setup_tables",
     "sg4_setup_term_grid", "sg4_setup_basis1d", "sg4_setup_destroy", "sg4_plan_create",
+    "sg4_plan_create_batch",
     "sg4_plan_set_stream", "sg4_plan_set_operator", "sg4_plan_set_henon_heiles", "sg4_apply",
     "sg4_apply_host", "sg4_apply_rows", "sg4_plan_stats", "sg4_plan_timing",
     "sg4_plan_kernel_times", "sg4_grad_V", "sg4_plan_set_field", "sg4_plan_autotune", "sg4_plan_config",
@@ -62,6 +63,7 @@ def lib():
         L.sg4_setup_basis1d.argtypes = [vp, i32, i32, C.POINTER(i32), C.POINTER(i32)] + [D_P] * 5
         L.sg4_setup_destroy.argtypes = [vp]
         L.sg4_plan_create.argtypes = [C.POINTER(Tables), i32, C.POINTER(vp)]
+        L.sg4_plan_create_batch.argtypes = [i32, C.POINTER(Tables), i32, C.POINTER(vp)]
         L.sg4_plan_set_stream.argtypes = [vp, vp]
         L.sg4_plan_set_operator.argtypes = [vp, C.POINTER(Operator)]
         L.sg4_plan_set_henon_heiles.argtypes = [vp, C.c_double]

@@ -61,7 +61,7 @@ struct TermDev {
   int32_t nb, nq;  // product sizes
   int32_t nact, pad;
   int16_t mat[MAXACT];   // MatDev index of each active mode
-  int8_t kmode[MAXACT];  // mode number of each active mode
+  int16_t kmode[MAXACT]; // index of the mode's kinetic coefficient (ck)
 };
 
 struct Item {
@@ -683,7 +683,12 @@ struct LaunchCfg {
 }  // namespace sg4
 
 struct sg4_plan {
-  int D = 0, Lmax = 0;
+  // one or several independent SG4 instances (a batch): their terms share the
+  // work-item list and their packed rows are concatenated
+  struct Inst { int D, Lmax; int64_t nterms, nb_ba, term0, row0, md0, ck0, lev0; };
+  std::vector<Inst> inst;
+  int nck = 0;          // kinetic coefficients: sum of the instances' D
+  int D = 0, Lmax = 0;  // of the (first) instance; type 10 needs a single instance
   int64_t nterms = 0, nb_ba = 0, sum_nb = 0, sum_nq = 0;
   int max_nvec = 0;
   int type_op = 0;
@@ -726,39 +731,43 @@ struct sg4_plan {
 namespace sg4 {
 
 static int build_terms(sg4_plan* P) {
-  const int D = P->D, L1 = P->Lmax + 1;
   P->h_terms.resize(P->nterms);
   std::vector<double> ck = P->h_ck;
-  if (ck.empty()) ck.assign(D, 0.0);
-  for (int64_t t = 0; t < P->nterms; ++t) {
-    TermDev T{};
-    T.boff = P->h_boff[t];
-    T.qoff = P->h_qoff[t];
-    T.nb = (int32_t)(P->h_boff[t + 1] - P->h_boff[t]);
-    T.nq = (int32_t)(P->h_qoff[t + 1] - P->h_qoff[t]);
-    double w = P->h_weights[t], vs = 0.0;
-    int nact = 0;
-    for (int k = 0; k < D; ++k) {
-      const int l = P->h_levels[(size_t)t * D + k];
-      const int idx = k * L1 + l;
-      const MatDev& m = P->h_md[idx];
-      if (m.nq == 1 && m.nb == 1 && P->type_op == 1) {
-        w *= m.b00 * m.btw00;
-        vs += ck[k] * m.d200;
-      } else {
-        if (nact >= MAXACT) {
-          set_error("more than 16 modes with more than one basis function in a term");
-          return SG4_ERR_UNSUPPORTED;
+  if ((int)ck.size() < P->nck) ck.assign(P->nck, 0.0);
+  for (const sg4_plan::Inst& I : P->inst) {
+    const int D = I.D, L1 = I.Lmax + 1;
+    for (int64_t tl = 0; tl < I.nterms; ++tl) {
+      const int64_t t = I.term0 + tl;
+      TermDev T{};
+      T.boff = P->h_boff[t];
+      T.qoff = P->h_qoff[t];
+      T.nb = (int32_t)(P->h_boff[t + 1] - P->h_boff[t]);
+      T.nq = (int32_t)(P->h_qoff[t + 1] - P->h_qoff[t]);
+      double w = P->h_weights[t], vs = 0.0;
+      int nact = 0;
+      for (int k = 0; k < D; ++k) {
+        const int l = P->h_levels[(size_t)I.lev0 + (size_t)tl * D + k];
+        const int64_t idx = I.md0 + k * L1 + l;
+        const MatDev& m = P->h_md[idx];
+        if (m.nq == 1 && m.nb == 1 && P->type_op == 1) {
+          w *= m.b00 * m.btw00;
+          vs += ck[I.ck0 + k] * m.d200;
+        } else {
+          if (nact >= MAXACT) {
+            set_error("more than 16 modes with more than one basis function in a term");
+            return SG4_ERR_UNSUPPORTED;
+          }
+          if (idx > INT16_MAX) { set_error("too many 1-D bases in the batch"); return SG4_ERR_UNSUPPORTED; }
+          T.mat[nact] = (int16_t)idx;
+          T.kmode[nact] = (int16_t)(I.ck0 + k);
+          ++nact;
         }
-        T.mat[nact] = (int16_t)idx;
-        T.kmode[nact] = (int8_t)k;
-        ++nact;
       }
+      T.nact = nact;
+      T.wgt = w;
+      T.vshift = vs;
+      P->h_terms[t] = T;
     }
-    T.nact = nact;
-    T.wgt = w;
-    T.vshift = vs;
-    P->h_terms[t] = T;
   }
   int rc = P->d_terms.alloc(sizeof(TermDev) * P->h_terms.size());
   if (rc) return rc;
@@ -875,13 +884,13 @@ static int ensure_f32(sg4_plan* P) {
   int rc;
   if (!P->d_mats32.p && (rc = P->d_mats32.alloc(sizeof(float) * P->nmats))) return rc;
   if (!P->d_V32.p && (rc = P->d_V32.alloc(sizeof(float) * P->sum_nq))) return rc;
-  if (!P->d_ck32.p && (rc = P->d_ck32.alloc(sizeof(float) * P->D))) return rc;
+  if (!P->d_ck32.p && (rc = P->d_ck32.alloc(sizeof(float) * P->nck))) return rc;
   hipLaunchKernelGGL(k_to_f32, dim3(grid_blocks(P->nmats)), dim3(256), 0, P->stream,
                      P->d_mats.as<double>(), P->d_mats32.as<float>(), P->nmats);
   hipLaunchKernelGGL(k_to_f32, dim3(grid_blocks(P->sum_nq)), dim3(256), 0, P->stream,
                      P->d_V.as<double>(), P->d_V32.as<float>(), P->sum_nq);
   hipLaunchKernelGGL(k_to_f32, dim3(1), dim3(64), 0, P->stream, P->d_ck.as<double>(),
-                     P->d_ck32.as<float>(), (int64_t)P->D);
+                     P->d_ck32.as<float>(), (int64_t)P->nck);
   HIP_TRY(hipGetLastError());
   P->f32_valid = true;
   return SG4_OK;
@@ -1031,16 +1040,10 @@ int sg4_setup_destroy(sg4_setup* s) {
   return SG4_OK;
 }
 
-int sg4_plan_create(const sg4_tables* t, int32_t max_nvec, sg4_plan** out) {
-  if (!t || !out || !t->levels || !t->weights || !t->map || !t->nq || !t->nb || !t->mats) {
-    set_error("null table pointer");
-    return SG4_ERR_ARG;
-  }
-  if (t->D < 1 || t->D > SG4_MAX_D || t->Lmax < 0 || t->Lmax > SG4_MAX_L || t->nterms < 1 ||
-      t->nb_ba < 1 || t->nb_ba > INT32_MAX || max_nvec < 1 || max_nvec > 4096) {
-    set_error("table sizes out of range");
-    return SG4_ERR_ARG;
-  }
+int sg4_plan_create_batch(int32_t ninst, const sg4_tables* tabs, int32_t max_nvec,
+                          sg4_plan** out) {
+  if (!tabs || !out || ninst < 1) { set_error("null table pointer"); return SG4_ERR_ARG; }
+  if (max_nvec < 1 || max_nvec > 4096) { set_error("max_nvec out of range"); return SG4_ERR_ARG; }
   static std::once_flag attr_once;
   std::call_once(attr_once, [] {
     (void)hipFuncSetAttribute((const void*)k_hpsi_t1_f64,
@@ -1053,95 +1056,125 @@ int sg4_plan_create(const sg4_tables* t, int32_t max_nvec, sg4_plan** out) {
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
   });
   std::unique_ptr<sg4_plan> P(new sg4_plan);
-  P->D = t->D; P->Lmax = t->Lmax; P->nterms = t->nterms; P->nb_ba = t->nb_ba;
   P->max_nvec = max_nvec;
-  const int D = t->D, L1 = t->Lmax + 1;
-  P->h_nq.assign(t->nq, t->nq + (size_t)D * L1);
-  P->h_nb.assign(t->nb, t->nb + (size_t)D * L1);
-  // matrices
+  bool nodes = true, qws = true;
   int64_t off = 0, noff = 0;
-  P->h_md.resize((size_t)D * L1);
-  std::vector<int64_t> node_off((size_t)D * L1);
-  for (int k = 0; k < D; ++k)
-    for (int l = 0; l < L1; ++l) {
-      const int idx = k * L1 + l;
-      const int nq = P->h_nq[idx], nb = P->h_nb[idx];
-      if (nq < 1 || nb < 1 || nb > nq || nq > SG4_MAX_N) {
-        set_error("1-D sizes out of range (need 1 <= nb <= nq <= 64)");
-        return SG4_ERR_ARG;
+  std::vector<int64_t> node_off;
+  std::vector<double> matsR, nodev, qwv;
+  std::vector<int32_t> mapv;
+  P->h_boff.push_back(0);
+  P->h_qoff.push_back(0);
+  for (int ii = 0; ii < ninst; ++ii) {
+    const sg4_tables* t = &tabs[ii];
+    if (!t->levels || !t->weights || !t->map || !t->nq || !t->nb || !t->mats) {
+      set_error("null table pointer");
+      return SG4_ERR_ARG;
+    }
+    if (t->D < 1 || t->D > SG4_MAX_D || t->Lmax < 0 || t->Lmax > SG4_MAX_L || t->nterms < 1 ||
+        t->nb_ba < 1 || t->nb_ba > INT32_MAX) {
+      set_error("table sizes out of range");
+      return SG4_ERR_ARG;
+    }
+    nodes = nodes && t->nodes;
+    qws = qws && t->qweights;
+    sg4_plan::Inst I;
+    I.D = t->D; I.Lmax = t->Lmax; I.nterms = t->nterms; I.nb_ba = t->nb_ba;
+    I.term0 = P->nterms; I.row0 = P->nb_ba; I.md0 = (int64_t)P->h_md.size(); I.ck0 = P->nck;
+    I.lev0 = (int64_t)P->h_levels.size();
+    const int D = t->D, L1 = t->Lmax + 1;
+    const int64_t noff0 = noff;
+    for (int k = 0; k < D; ++k)
+      for (int l = 0; l < L1; ++l) {
+        const int nq = t->nq[k * L1 + l], nb = t->nb[k * L1 + l];
+        if (nq < 1 || nb < 1 || nb > nq || nq > SG4_MAX_N) {
+          set_error("1-D sizes out of range (need 1 <= nb <= nq <= 64)");
+          return SG4_ERR_ARG;
+        }
+        MatDev m{};
+        m.nq = nq; m.nb = nb;
+        m.B = off; off += (int64_t)nq * nb;
+        m.BtW = off; off += (int64_t)nq * nb;
+        m.D1 = off; off += (int64_t)nq * nq;
+        m.D2 = off; off += (int64_t)nq * nq;
+        P->h_md.push_back(m);
+        P->h_nq.push_back(nq);
+        P->h_nb.push_back(nb);
+        node_off.push_back(noff);
+        noff += nq;
       }
-      MatDev& m = P->h_md[idx];
-      m.nq = nq; m.nb = nb;
-      m.B = off; off += (int64_t)nq * nb;
-      m.BtW = off; off += (int64_t)nq * nb;
-      m.D1 = off; off += (int64_t)nq * nq;
-      m.D2 = off; off += (int64_t)nq * nq;
-      m.b00 = t->mats[m.B]; m.btw00 = t->mats[m.BtW]; m.d200 = t->mats[m.D2];
-      node_off[idx] = noff; noff += nq;
+    // row-major copies of this instance's matrices (the kernels read rows)
+    const int64_t base = P->h_md[I.md0].B;
+    matsR.resize(off);
+    for (int64_t i = I.md0; i < (int64_t)P->h_md.size(); ++i) {
+      MatDev& m = P->h_md[i];
+      auto tr = [&](int64_t o, int rows, int cols) {
+        for (int r = 0; r < rows; ++r)
+          for (int c = 0; c < cols; ++c)
+            matsR[o + (int64_t)r * cols + c] = t->mats[o - base + r + (int64_t)rows * c];
+      };
+      tr(m.B, m.nq, m.nb);
+      tr(m.BtW, m.nb, m.nq);
+      tr(m.D1, m.nq, m.nq);
+      tr(m.D2, m.nq, m.nq);
+      m.b00 = t->mats[m.B - base]; m.btw00 = t->mats[m.BtW - base]; m.d200 = t->mats[m.D2 - base];
     }
-  // terms
-  P->h_levels.assign(t->levels, t->levels + (size_t)D * t->nterms);
-  P->h_weights.assign(t->weights, t->weights + t->nterms);
-  P->h_boff.resize(t->nterms + 1);
-  P->h_qoff.resize(t->nterms + 1);
-  P->h_boff[0] = P->h_qoff[0] = 0;
-  for (int64_t tt = 0; tt < t->nterms; ++tt) {
-    int64_t nbt = 1, nqt = 1;
-    for (int k = 0; k < D; ++k) {
-      const int l = P->h_levels[(size_t)tt * D + k];
-      if (l < 0 || l > t->Lmax) { set_error("term level out of range"); return SG4_ERR_ARG; }
-      nbt *= P->h_nb[k * L1 + l];
-      nqt *= P->h_nq[k * L1 + l];
+    if (t->nodes) nodev.insert(nodev.end(), t->nodes, t->nodes + (noff - noff0));
+    if (t->qweights) qwv.insert(qwv.end(), t->qweights, t->qweights + (noff - noff0));
+    P->h_levels.insert(P->h_levels.end(), t->levels, t->levels + (size_t)D * t->nterms);
+    P->h_weights.insert(P->h_weights.end(), t->weights, t->weights + t->nterms);
+    int64_t nbsum = 0;
+    for (int64_t tt = 0; tt < t->nterms; ++tt) {
+      int64_t nbt = 1, nqt = 1;
+      for (int k = 0; k < D; ++k) {
+        const int l = t->levels[(size_t)tt * D + k];
+        if (l < 0 || l > t->Lmax) { set_error("term level out of range"); return SG4_ERR_ARG; }
+        nbt *= t->nb[k * L1 + l];
+        nqt *= t->nq[k * L1 + l];
+      }
+      if (nqt > INT32_MAX / 64) { set_error("term too large"); return SG4_ERR_UNSUPPORTED; }
+      P->h_boff.push_back(P->h_boff.back() + nbt);
+      P->h_qoff.push_back(P->h_qoff.back() + nqt);
+      nbsum += nbt;
     }
-    if (nqt > INT32_MAX / 64) { set_error("term too large"); return SG4_ERR_UNSUPPORTED; }
-    P->h_boff[tt + 1] = P->h_boff[tt] + nbt;
-    P->h_qoff[tt + 1] = P->h_qoff[tt] + nqt;
+    for (int64_t e = 0; e < nbsum; ++e) {
+      const int32_t m = t->map[e];
+      if (m < -1 || m >= t->nb_ba) { set_error("map entry out of range"); return SG4_ERR_ARG; }
+      if (I.row0 + t->nb_ba > INT32_MAX) { set_error("batch exceeds int32 rows"); return SG4_ERR_UNSUPPORTED; }
+      mapv.push_back(m < 0 ? -1 : (int32_t)(m + I.row0));
+    }
+    P->nterms += t->nterms;
+    P->nb_ba += t->nb_ba;
+    P->nck += D;
+    P->inst.push_back(I);
   }
+  P->D = tabs[0].D;
+  P->Lmax = tabs[0].Lmax;
   P->sum_nb = P->h_boff.back();
   P->sum_nq = P->h_qoff.back();
-  for (int64_t e = 0; e < P->sum_nb; ++e)
-    if (t->map[e] < -1 || t->map[e] >= t->nb_ba) { set_error("map entry out of range"); return SG4_ERR_ARG; }
-  int rc;
   P->nmats = off;
-  // the kernels read the 1-D matrices row by row: store row-major copies at
-  // the same offsets
-  std::vector<double> matsR(off);
-  for (const MatDev& m : P->h_md) {
-    auto tr = [&](int64_t o, int rows, int cols) {
-      for (int i = 0; i < rows; ++i)
-        for (int j = 0; j < cols; ++j) matsR[o + (int64_t)i * cols + j] = t->mats[o + i + (int64_t)rows * j];
-    };
-    tr(m.B, m.nq, m.nb);
-    tr(m.BtW, m.nb, m.nq);
-    tr(m.D1, m.nq, m.nq);
-    tr(m.D2, m.nq, m.nq);
-  }
+  int rc;
   if ((rc = P->d_mats.alloc(sizeof(double) * off))) return rc;
   HIP_TRY(hipMemcpy(P->d_mats.p, matsR.data(), sizeof(double) * off, hipMemcpyHostToDevice));
   if ((rc = P->d_md.alloc(sizeof(MatDev) * P->h_md.size()))) return rc;
   HIP_TRY(hipMemcpy(P->d_md.p, P->h_md.data(), sizeof(MatDev) * P->h_md.size(), hipMemcpyHostToDevice));
   if ((rc = P->d_map.alloc(sizeof(int32_t) * P->sum_nb))) return rc;
-  HIP_TRY(hipMemcpy(P->d_map.p, t->map, sizeof(int32_t) * P->sum_nb, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(P->d_map.p, mapv.data(), sizeof(int32_t) * P->sum_nb, hipMemcpyHostToDevice));
   if ((rc = P->d_levels.alloc(sizeof(int32_t) * P->h_levels.size()))) return rc;
   HIP_TRY(hipMemcpy(P->d_levels.p, P->h_levels.data(), sizeof(int32_t) * P->h_levels.size(), hipMemcpyHostToDevice));
   if ((rc = P->d_qoff.alloc(sizeof(int64_t) * P->h_qoff.size()))) return rc;
   HIP_TRY(hipMemcpy(P->d_qoff.p, P->h_qoff.data(), sizeof(int64_t) * P->h_qoff.size(), hipMemcpyHostToDevice));
   if ((rc = P->d_nqtab.alloc(sizeof(int32_t) * P->h_nq.size()))) return rc;
   HIP_TRY(hipMemcpy(P->d_nqtab.p, P->h_nq.data(), sizeof(int32_t) * P->h_nq.size(), hipMemcpyHostToDevice));
-  if (t->nodes) {
+  if ((rc = P->d_node_off.alloc(sizeof(int64_t) * node_off.size()))) return rc;
+  HIP_TRY(hipMemcpy(P->d_node_off.p, node_off.data(), sizeof(int64_t) * node_off.size(), hipMemcpyHostToDevice));
+  if (nodes) {
     if ((rc = P->d_nodes.alloc(sizeof(double) * noff))) return rc;
-    HIP_TRY(hipMemcpy(P->d_nodes.p, t->nodes, sizeof(double) * noff, hipMemcpyHostToDevice));
-    if ((rc = P->d_node_off.alloc(sizeof(int64_t) * node_off.size()))) return rc;
-    HIP_TRY(hipMemcpy(P->d_node_off.p, node_off.data(), sizeof(int64_t) * node_off.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P->d_nodes.p, nodev.data(), sizeof(double) * noff, hipMemcpyHostToDevice));
     P->have_nodes = true;
   }
-  if (t->qweights) {
+  if (qws) {
     if ((rc = P->d_qw.alloc(sizeof(double) * noff))) return rc;
-    HIP_TRY(hipMemcpy(P->d_qw.p, t->qweights, sizeof(double) * noff, hipMemcpyHostToDevice));
-    if (!P->d_node_off.p) {
-      if ((rc = P->d_node_off.alloc(sizeof(int64_t) * node_off.size()))) return rc;
-      HIP_TRY(hipMemcpy(P->d_node_off.p, node_off.data(), sizeof(int64_t) * node_off.size(), hipMemcpyHostToDevice));
-    }
+    HIP_TRY(hipMemcpy(P->d_qw.p, qwv.data(), sizeof(double) * noff, hipMemcpyHostToDevice));
     P->have_qw = true;
   }
   if ((rc = P->d_V.alloc(sizeof(double) * P->sum_nq))) return rc;
@@ -1149,17 +1182,24 @@ int sg4_plan_create(const sg4_tables* t, int32_t max_nvec, sg4_plan** out) {
   const size_t blk = sizeof(double) * (size_t)P->nb_ba * max_nvec;
   if ((rc = P->d_rows_in.alloc(blk))) return rc;
   if ((rc = P->d_rows_out.alloc(blk))) return rc;
-  // algorithmic flop / byte counts (type 1): per vector 3 * sum_t S_t,
-  // S_t = sum_k 2 N_t n_k; bytes: V grid + map (+ 16 nb_ba per vector)
+  // algorithmic flop / byte counts: per vector sum_t S_t with
+  // S_t = sum_k 2 N_t n_k (x3 for type 1, x4 for type 10); bytes: V grid + map
   double S = 0;
-  for (int64_t tt = 0; tt < t->nterms; ++tt) {
-    const double Nt = (double)(P->h_qoff[tt + 1] - P->h_qoff[tt]);
-    for (int k = 0; k < D; ++k) S += 2.0 * Nt * P->h_nq[k * L1 + P->h_levels[(size_t)tt * D + k]];
-  }
-  P->flops_per_vec = S;   // sum_t S_t; x3 for type 1, x4 for type 10
+  for (const sg4_plan::Inst& I : P->inst)
+    for (int64_t tl = 0; tl < I.nterms; ++tl) {
+      const int64_t tt = I.term0 + tl;
+      const double Nt = (double)(P->h_qoff[tt + 1] - P->h_qoff[tt]);
+      for (int k = 0; k < I.D; ++k)
+        S += 2.0 * Nt * P->h_nq[I.md0 + k * (I.Lmax + 1) + P->h_levels[I.lev0 + tl * I.D + k]];
+    }
+  P->flops_per_vec = S;
   P->bytes_fixed = 8.0 * P->sum_nq + 4.0 * P->sum_nb;
   *out = P.release();
   return SG4_OK;
+}
+
+int sg4_plan_create(const sg4_tables* t, int32_t max_nvec, sg4_plan** out) {
+  return sg4_plan_create_batch(1, t, max_nvec, out);
 }
 
 int sg4_plan_set_stream(sg4_plan* p, void* s) {
@@ -1173,11 +1213,12 @@ int sg4_plan_set_operator(sg4_plan* p, const sg4_operator* op) {
   int rc;
   if (op->type_op == 1) {
     if (!op->ck) { set_error("type 1 needs ck"); return SG4_ERR_ARG; }
-    p->h_ck.assign(op->ck, op->ck + p->D);
-    if ((rc = p->d_ck.alloc(sizeof(double) * p->D))) return rc;
-    HIP_TRY(hipMemcpy(p->d_ck.p, op->ck, sizeof(double) * p->D, hipMemcpyHostToDevice));
+    p->h_ck.assign(op->ck, op->ck + p->nck);
+    if ((rc = p->d_ck.alloc(sizeof(double) * p->nck))) return rc;
+    HIP_TRY(hipMemcpy(p->d_ck.p, op->ck, sizeof(double) * p->nck, hipMemcpyHostToDevice));
   } else if (op->type_op == 10) {
     if (!op->G || !op->Jac || !op->sqRJ) { set_error("type 10 needs G, Jac and sqRJ"); return SG4_ERR_ARG; }
+    if (p->inst.size() != 1) { set_error("type 10 needs a single-instance plan"); return SG4_ERR_UNSUPPORTED; }
     const size_t n = (size_t)p->sum_nq;
     if ((rc = p->d_G.alloc(sizeof(double) * n * p->D * p->D))) return rc;
     if ((rc = p->d_Jac.alloc(sizeof(double) * n))) return rc;
@@ -1185,7 +1226,7 @@ int sg4_plan_set_operator(sg4_plan* p, const sg4_operator* op) {
     HIP_TRY(hipMemcpy(p->d_G.p, op->G, sizeof(double) * n * p->D * p->D, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_Jac.p, op->Jac, sizeof(double) * n, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_sqRJ.p, op->sqRJ, sizeof(double) * n, hipMemcpyHostToDevice));
-    p->h_ck.assign(p->D, 0.0);
+    p->h_ck.assign(p->nck, 0.0);
   } else {
     set_error("type_op must be 1 or 10");
     return SG4_ERR_UNSUPPORTED;
@@ -1201,23 +1242,25 @@ int sg4_plan_set_operator(sg4_plan* p, const sg4_operator* op) {
 int sg4_plan_set_henon_heiles(sg4_plan* p, double lambda) {
   if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
   if (!p->have_nodes) { set_error("plan was created without grid nodes"); return SG4_ERR_STATE; }
-  HHArgs a;
-  a.levels = p->d_levels.as<int32_t>();
-  a.qoff = p->d_qoff.as<int64_t>();
-  a.nodes = p->d_nodes.as<double>();
-  a.node_off = p->d_node_off.as<int64_t>();
-  a.nqtab = p->d_nqtab.as<int32_t>();
-  a.D = p->D; a.Lmax = p->Lmax; a.nterms = p->nterms; a.lambda = lambda;
-  a.V = p->d_V.as<double>();
-  hipLaunchKernelGGL(k_grid_hh, dim3((unsigned)std::min<int64_t>(p->nterms, 65535)), dim3(256), 0,
-                     p->stream, a);
+  for (const sg4_plan::Inst& I : p->inst) {
+    HHArgs a;
+    a.levels = p->d_levels.as<int32_t>() + I.lev0;
+    a.qoff = p->d_qoff.as<int64_t>() + I.term0;
+    a.nodes = p->d_nodes.as<double>();
+    a.node_off = p->d_node_off.as<int64_t>() + I.md0;
+    a.nqtab = p->d_nqtab.as<int32_t>() + I.md0;
+    a.D = I.D; a.Lmax = I.Lmax; a.nterms = I.nterms; a.lambda = lambda;
+    a.V = p->d_V.as<double>();
+    hipLaunchKernelGGL(k_grid_hh, dim3((unsigned)std::min<int64_t>(I.nterms, 65535)), dim3(256), 0,
+                       p->stream, a);
+  }
   HIP_TRY(hipGetLastError());
   p->type_op = 1;
   (void)p->d_V0.alloc(0);
-  p->h_ck.assign(p->D, -0.5);
+  p->h_ck.assign(p->nck, -0.5);
   int rc;
-  if ((rc = p->d_ck.alloc(sizeof(double) * p->D))) return rc;
-  HIP_TRY(hipMemcpy(p->d_ck.p, p->h_ck.data(), sizeof(double) * p->D, hipMemcpyHostToDevice));
+  if ((rc = p->d_ck.alloc(sizeof(double) * p->nck))) return rc;
+  HIP_TRY(hipMemcpy(p->d_ck.p, p->h_ck.data(), sizeof(double) * p->nck, hipMemcpyHostToDevice));
   p->items.clear();
   p->f32_valid = false;
   rc = build_terms(p);

@@ -187,15 +187,24 @@ def propagate_field(prop, psi0, eps, steps_per_segment, store_every=None, npolys
     return psi, cps, npolys
 
 
-def overlap_gradient(prop, psi0, target, eps, steps_per_segment, checkpoint_every=10):
-    """J = |<target|psi(T)>|^2 and dJ/deps_s for all segments (exact for the
-    discrete Chebyshev map with the polynomial orders of the forward run)."""
+def overlap_gradient(prop, psi0, target, eps, steps_per_segment, checkpoint_every=10,
+                     blocks=None):
+    """J = sum_i |<target_i|psi_i(T)>|^2 over the row blocks of a batch
+    (blocks = [(r0, r1), ...]; default: the whole vector) and dJ/deps_s for
+    all segments, exact for the discrete Chebyshev map with the polynomial
+    orders of the forward run.  The adjoint seed is sum_i a_i target_i with
+    a_i = <target_i|psi_i(T)>, so that dJ/deps_s = 2 Re <lambda_s|dpsi_s>."""
     S = len(eps)
     psiT, cps, npolys = propagate_field(prop, psi0, eps, steps_per_segment, checkpoint_every)
-    a = torch.sum(target.conj() * psiT).item()
-    J = abs(a) ** 2
+    if blocks is None:
+        blocks = [(0, psiT.shape[-1])]
+    lam = torch.zeros_like(target)
+    J = 0.0
+    for r0, r1 in blocks:
+        ai = torch.sum(target[..., r0:r1].conj() * psiT[..., r0:r1]).item()
+        J += abs(ai) ** 2
+        lam[..., r0:r1] = ai * target[..., r0:r1]
     grad = np.zeros(S)
-    lam = target.clone()
     seg_states = {}
     for s in range(S - 1, -1, -1):
         if s not in seg_states:   # recompute the states of this checkpoint block
@@ -213,7 +222,7 @@ def overlap_gradient(prop, psi0, target, eps, steps_per_segment, checkpoint_ever
         dpsi = torch.zeros_like(psi)
         for j in range(steps_per_segment):
             psi, dpsi = prop.step_tangent(psi, dpsi, npolys[s][j])
-        grad[s] = 2.0 * (np.conj(a) * torch.sum(lam.conj() * dpsi).item()).real
+        grad[s] = 2.0 * torch.sum(lam.conj() * dpsi).item().real
         for j in range(steps_per_segment - 1, -1, -1):
             lam, _ = prop.step(lam, conj=True, npoly=npolys[s][j])
     prop.set_field(0.0)

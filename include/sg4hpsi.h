@@ -120,6 +120,14 @@ int sg4_setup_destroy(sg4_setup *s);
  * pre-sized work buffers for blocks of up to max_nvec vectors: no device
  * allocation happens inside sg4_apply.                                       */
 int sg4_plan_create(const sg4_tables *tables, int32_t max_nvec, sg4_plan **out);
+/* A batch of ninst independent SG4 operators (different D, L, grids) in one
+ * plan: the operator is block-diagonal over the concatenated packed rows
+ * (instance i owns rows [row0_i, row0_i + nb_ba_i) in instance order), and
+ * one apply runs every term of every instance in a single launch.  Operator
+ * grids and kinetic coefficients are the instances' arrays concatenated in
+ * the same order.  No counterpart in the reference (one operator per run). */
+int sg4_plan_create_batch(int32_t ninst, const sg4_tables *tables, int32_t max_nvec,
+                          sg4_plan **out);
 /* hip_stream: a hipStream_t (NULL = default stream) */
 int sg4_plan_set_stream(sg4_plan *p, void *hip_stream);
 /* copy the operator grids to the device (host pointers) */
