@@ -51,7 +51,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--prec", default="fp64", choices=list(PREC))
+    ap.add_argument("--prec", default="fp64", choices=list(PREC),
+                    help="precision of the reported value; the other mode is timed beside it")
+    ap.add_argument("--autotune", action="store_true", help="autotune launch parameters first")
+    ap.add_argument("--no-other", action="store_true", help="do not time the other precision")
     ap.add_argument("--D", type=int, default=10)
     ap.add_argument("--L", type=int, default=8)
     ap.add_argument("--nvec", type=int, default=16)
@@ -76,21 +79,35 @@ def main():
     Xc = X.T.contiguous()                            # (nvec, nb_ba) = column-major block
     Y = torch.empty_like(Xc)
     prec = PREC[args.prec]
-    for _ in range(args.warmup):
-        op.apply(Xc, Y, prec)
-    torch.cuda.synchronize()
-    op.timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        op.apply(Xc, Y, prec)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    kt = op.kernel_times()
-    op.timing(False)
+    if args.autotune:
+        op.autotune(nvec, prec)
+
+    def timed(p, steps, warmup):
+        for _ in range(warmup):
+            op.apply(Xc, Y, p)
+        torch.cuda.synchronize()
+        op.timing(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            op.apply(Xc, Y, p)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        kt = op.kernel_times()
+        op.timing(False)
+        return dt, float(np.mean(kt)) * 1e-3
+
+    dt, kavg = timed(prec, args.steps, args.warmup)
     st = op.stats(nvec)
-    kavg = float(np.mean(kt)) * 1e-3
     achieved = st["flops"] / kavg / 1e12
+    other = None
+    if not args.no_other:
+        op_name = "fp64" if args.prec != "fp64" else "f32"
+        if args.autotune:
+            op.autotune(nvec, PREC[op_name])
+        dto, kavgo = timed(PREC[op_name], max(3, args.steps // 2), 2)
+        other = {"precision": op_name, "H·psi/s": nvec * max(3, args.steps // 2) / dto,
+                 "kernel_ms": kavgo * 1e3}
     line = {
         "metric": "H·psi/s 10D Smolyak L=8 block-16: split-FP32 MFMA vs FP64, @1e-8 rel eigen err",
         "value": nvec * args.steps / dt,
@@ -106,13 +123,18 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{D}D Henon-Heiles SG4 Smolyak L={L} block-{nvec} H.psi",
                    "D": D, "L": L, "nvec": nvec, "nterms": b.nterms, "nb_ba": b.nb_ba,
-                   "precision": args.prec, "items": st["items"]},
+                   "precision": args.prec, "items": st["items"],
+                   "launch": op.config(nvec, prec)},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK[args.prec],
                      "unit": "TFLOP/s", "frac": achieved / PEAK[args.prec], "traffic": None,
-                     "kernel": "k_hpsi_t1_f64", "kernel_ms": kavg * 1e3,
+                     "kernel": "k_hpsi_t1_f64" if args.prec == "fp64" else "k_hpsi_t1_f32",
+                     "kernel_ms": kavg * 1e3,
                      "flops_per_launch": st["flops"], "algorithmic_bytes": st["bytes"],
                      "hbm_GBps_algorithmic": st["bytes"] / kavg / 1e9},
     }
+    if other is not None:
+        line["other_precision"] = other
+        line["speedup_vs_other"] = (nvec * args.steps / dt) / other["H·psi/s"]
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(D, L, nvec, args.cpu_stride)
     print(json.dumps(line), flush=True)

@@ -716,6 +716,7 @@ struct sg4_plan {
   struct GraphEntry { int nvec, prec; const double* psi; double* hpsi; hipGraphExec_t exec; };
   std::vector<GraphEntry> graphs;
   double flops_per_vec = 0, bytes_fixed = 0;
+  int64_t last_items = -1;   // work items of the last apply
   // optional kernel timing: HIP events around the H.psi launch of each apply
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -946,6 +947,7 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
   a.items = ip->items.as<Item>();
   a.scratch = ip->scratch.as<double>();
   a.slot_size = ip->slot_size;
+  P->last_items = ip->nitems;
   if (P->timing) {
     if (P->ev_used == P->ev_pairs.size()) {
       hipEvent_t e0, e1;
@@ -1313,8 +1315,7 @@ int sg4_plan_stats(const sg4_plan* p, int32_t nvec, double* flops, double* bytes
   if (flops) *flops = (p->type_op == 10 ? 4.0 : 3.0) * p->flops_per_vec * nvec;
   if (bytes) *bytes = p->bytes_fixed + 16.0 * (double)p->nb_ba * nvec;
   if (items) {
-    auto it8 = p->items.find(nvec * 16 + 8);
-    *items = (it8 == p->items.end()) ? -1 : it8->second->nitems;
+    *items = p->last_items;
   }
   return SG4_OK;
 }
