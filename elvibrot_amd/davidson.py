@@ -18,9 +18,13 @@ Mirrors the control flow of sub_propagation_Davidson
 
 All block algebra (projections, residuals, Gram-Schmidt, norms) is FP64 on
 the device; the small projected eigenproblem is solved on the host.  With
-``prec`` = PREC_F32 the basis is expanded with the FP32 H.psi; the converged
-block then gets one FP64 Rayleigh-Ritz (an FP64 H.psi on the Ritz vectors),
-which makes the eigenvalue error second order in the FP32 error.
+``prec`` = PREC_F32 the iterations run with the FP32 H.psi until every
+residual is below ``switch_resi`` (above the FP32 noise floor of the
+residuals); the basis is then collapsed onto the current Ritz vectors, H is
+re-applied to them in FP64 and the iterations continue in FP64 to the
+requested tolerances.  The FP32 phase does the bulk of the subspace
+expansion; the FP64 phase (a few iterations) removes the FP32 error from the
+eigenpairs.
 """
 import numpy as np
 import torch
@@ -56,7 +60,8 @@ class DavidsonResult:
 
 def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
                              conv_ene=1e-4, block=None, prec=PREC_FP64, psi0=None,
-                             final_rr_fp64=True, device="cuda:0", verbose=False):
+                             final_rr_fp64=True, device="cuda:0", verbose=False,
+                             switch_resi=None):
     """Lowest ``nb_WP`` eigenpairs of the SG4 operator ``op``.
 
     Returns a DavidsonResult with E (host numpy), X (device, nb_WP x nb_ba),
@@ -110,6 +115,9 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
                 keep.append(w)
         return torch.stack(keep) if keep else None
 
+    phase_prec = prec
+    switch_resi = max(conv_resi, 2e-4) if switch_resi is None else switch_resi
+    n_low = 0
     it = 0
     for it in range(max_it + 1):
         W = orthonormalize(new, ndim)
@@ -126,7 +134,7 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
                 break
             m = W.shape[0]
         Vb[ndim:ndim + m] = W
-        HV[ndim:ndim + m] = apply(W, prec)
+        HV[ndim:ndim + m] = apply(W, phase_prec)
         ndim += m
         Hp = Vb[:ndim] @ HV[:ndim].T
         Hp = 0.5 * (Hp + Hp.T)
@@ -144,7 +152,27 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
         if verbose:
             print(f"davidson it {it} ndim {ndim} max|r| {rn.max():.2e} max|dE| {dE.max():.2e} conv {conv.sum()}")
         Ene0 = E.copy()
-        if conv.all():
+        if phase_prec != PREC_FP64 and rn.max() < switch_resi:
+            # FP64 phase: restart on the Ritz vectors with an FP64 H.psi
+            n_low = n_hpsi
+            keep = min(ndim, nb_WP + block)
+            Q, _ = torch.linalg.qr(X_all[:keep].T)
+            Vb[:keep] = Q.T
+            HV[:keep] = apply(Vb[:keep].contiguous(), PREC_FP64)
+            ndim = keep
+            phase_prec = PREC_FP64
+            Hp = Vb[:ndim] @ HV[:ndim].T
+            ev, vec = np.linalg.eigh(0.5 * (Hp + Hp.T).cpu().numpy())
+            vec_t = torch.tensor(vec, device=dev)
+            X_all = vec_t.T @ Vb[:ndim]
+            HX_all = vec_t.T @ HV[:ndim]
+            E = ev[:nb_WP]
+            X = X_all[:nb_WP]
+            R = HX_all[:nb_WP] - torch.tensor(E, device=dev)[:, None] * X
+            rn = torch.linalg.norm(R, dim=1).cpu().numpy()
+            conv = np.zeros(nb_WP, dtype=bool)
+            Ene0 = E.copy()
+        elif conv.all() and phase_prec == PREC_FP64:
             break
         todo = np.nonzero(~conv)[0][:block]
         Rt = R[todo]
@@ -152,16 +180,7 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
         Dg = Et - E0[None, :]
         Dg = torch.where(Dg.abs() > conv_resi, Dg, Dg + 1e-3)
         new = Rt / Dg
-    E_fp64 = None
-    if prec != PREC_FP64 and final_rr_fp64 and X is not None:
-        nr = min(ndim, nb_WP + block)
-        Q = X_all[:nr]
-        Q, _ = torch.linalg.qr(Q.T)
-        Q = Q.T.contiguous()
-        HQ = apply(Q, PREC_FP64)
-        Hq = Q @ HQ.T
-        ev, vec = np.linalg.eigh(0.5 * (Hq + Hq.T).cpu().numpy())
-        E_fp64 = ev[:nb_WP]
-        X = (torch.tensor(vec[:, :nb_WP], device=dev).T @ Q)
-        E = E_fp64
-    return DavidsonResult(E, X, it, n_hpsi, conv, E_fp64, hist)
+    E_fp64 = E if prec != PREC_FP64 else None
+    res = DavidsonResult(E, X, it, n_hpsi, conv, E_fp64, hist)
+    res.n_hpsi_low = n_low      # H.psi vector applications done in the reduced precision
+    return res

@@ -31,7 +31,7 @@ import numpy as np
 import torch
 from scipy.special import jv
 
-from . import PREC_FP64, SG4Operator
+from . import PREC_FP64, SG4Operator, SG4Batch
 
 
 def cheby_coefficients(alpha, n):
@@ -83,9 +83,16 @@ class ChebyshevPropagator:
     def mu_s(self, x):
         """(dHs/deps) x = mu x / Esc with the SG4 representation of mu."""
         if self.op_mu is None:
-            b = self.op.basis
-            self.op_mu = SG4Operator(b, max_nvec=self.op.max_nvec,
-                                     potential=self.mu_grid.cpu().numpy(), ck=np.zeros(b.D))
+            mu = self.mu_grid.cpu().numpy()
+            if hasattr(self.op, "bases"):      # batch: one mu grid per instance
+                q0 = self.op.q0
+                self.op_mu = SG4Batch(self.op.bases, max_nvec=self.op.max_nvec,
+                                      potential=[mu[q0[i]:q0[i + 1]] for i in range(len(q0) - 1)],
+                                      ck=[np.zeros(b.D) for b in self.op.bases])
+            else:
+                b = self.op.basis
+                self.op_mu = SG4Operator(b, max_nvec=self.op.max_nvec, potential=mu,
+                                         ck=np.zeros(b.D))
         return self._apply_c(self.op_mu, x) / self.Esc
 
     def set_field(self, eps):

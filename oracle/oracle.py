@@ -55,6 +55,8 @@ def lib():
         L.orc_sg_grid_hh.argtypes = [C.c_void_p, C.c_double, D_P]
         L.orc_hpsi_range.argtypes = [C.c_void_p, C.c_int, D_P, D_P, D_P, D_P, D_P, C.c_int,
                                      D_P, D_P, C.c_int64, C.c_int64, C.c_int64]
+        L.orc_hpsi_absbound.argtypes = [C.c_void_p, C.c_int, D_P, D_P, D_P, D_P, D_P, C.c_int,
+                                        D_P, D_P]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -185,6 +187,22 @@ class SG4:
                                   nvec, _d(pf), _d(out), t0, t1, tstride)
         assert rc == 0
         return out[:, 0] if one else out
+
+
+def _absbound(self, psi, V, ck=None):
+    """sum_t |W_t r_t| per element (condition of the Smolyak scatter-sum)."""
+    psi = np.asarray(psi, dtype=np.float64)
+    one = psi.ndim == 1
+    pf = np.asfortranarray(psi[:, None] if one else psi)
+    out = np.zeros((self.nb_ba, pf.shape[1]), order="F")
+    c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+    V, ck = c(V), c(ck)
+    assert lib().orc_hpsi_absbound(self._h, 1, _d(V), _d(ck), None, None, None, pf.shape[1],
+                                   _d(pf), _d(out)) == 0
+    return out[:, 0] if one else out
+
+
+SG4.absbound = _absbound
 
 
 def num_threads():

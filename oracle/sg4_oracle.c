@@ -667,10 +667,10 @@ static void term_apply(const orc_sg *s, int64_t t, const orc_op *op,
  * scatter-add W_t * r_t (tabR_AT_iG_TO_tabPackedBasis :1255-1294).
  * psi, hpsi: nb_ba x nvec, column-major.  Terms in [t0, t1) only (t1<0: all);
  * the whole output is still zeroed first. */
-int orc_hpsi_range(const orc_sg *s, int type, const double *V, const double *ck,
-                   const double *G, const double *Jac, const double *sqRJ,
-                   int nvec, const double *psi, double *hpsi, int64_t t0,
-                   int64_t t1, int64_t tstride) {
+static int hpsi_impl(const orc_sg *s, int type, const double *V, const double *ck,
+                     const double *G, const double *Jac, const double *sqRJ,
+                     int nvec, const double *psi, double *hpsi, int64_t t0,
+                     int64_t t1, int64_t tstride, int absmode) {
   orc_op op = {type, V, ck, G, Jac, sqRJ};
   if (t1 < 0 || t1 > s->nterms) t1 = s->nterms;
   if (tstride < 1) tstride = 1;
@@ -707,7 +707,7 @@ int orc_hpsi_range(const orc_sg *s, int type, const double *V, const double *ck,
         term_apply(s, t, &op, pt, rt, w1, w2, w3, w4);
         double *h = mine + (size_t)nb_ba * v;
         for (int64_t e = 0; e < nbt; e++)
-          if (mp[e] >= 0) h[mp[e]] += s->weight[t] * rt[e];
+          if (mp[e] >= 0) h[mp[e]] += absmode ? fabs(s->weight[t] * rt[e]) : s->weight[t] * rt[e];
       }
     }
     free(w1); free(w2); free(w3); free(w4); free(pt); free(rt);
@@ -718,6 +718,22 @@ int orc_hpsi_range(const orc_sg *s, int type, const double *V, const double *ck,
   }
   free(acc);
   return 0;
+}
+
+int orc_hpsi_range(const orc_sg *s, int type, const double *V, const double *ck,
+                   const double *G, const double *Jac, const double *sqRJ,
+                   int nvec, const double *psi, double *hpsi, int64_t t0,
+                   int64_t t1, int64_t tstride) {
+  return hpsi_impl(s, type, V, ck, G, Jac, sqRJ, nvec, psi, hpsi, t0, t1, tstride, 0);
+}
+
+/* Condition of the weighted scatter-sum: sum_t |W_t r_t| per packed element
+ * (same term results as orc_hpsi).  The rounding error of any summation
+ * order of H psi is bounded by a small multiple of eps times this. */
+int orc_hpsi_absbound(const orc_sg *s, int type, const double *V, const double *ck,
+                      const double *G, const double *Jac, const double *sqRJ,
+                      int nvec, const double *psi, double *bound) {
+  return hpsi_impl(s, type, V, ck, G, Jac, sqRJ, nvec, psi, bound, 0, -1, 1, 1);
 }
 
 int orc_hpsi(const orc_sg *s, int type, const double *V, const double *ck,

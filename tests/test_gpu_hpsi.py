@@ -79,12 +79,23 @@ def test_fp64_general_potential_and_coefficients(cuda):
 
 
 def test_10d_L8_parity_two_vectors(cuda):
+    """At 10D L=8 the Smolyak scatter-sum is ill-conditioned:
+    max_e sum_t |W_t r_t| / max|H psi| ~ 5e4, so eps times it (~1e-11) bounds
+    the max-norm agreement of ANY two FP64 summation orders.  Parity is
+    therefore checked as a forward-error bound per element,
+    |got - ref| <= 1e-13 sum_t |W_t r_t| (~450 eps), plus 2-norm relative
+    error < 1e-12."""
     D, L = 10, 8
     b, op, o, V = hh_pair(D, L, 16)
     psi = np.random.default_rng(2026).standard_normal((b.nb_ba, 2))
     got = op.apply_host(psi)
     ref = o.hpsi(psi, V, -0.5 * np.ones(D))
-    assert rel(got, ref) < TOL
+    bound = o.absbound(psi, V, -0.5 * np.ones(D))
+    r = np.max(np.abs(got - ref) / bound)
+    print(f"10D max-norm rel {rel(got, ref):.2e}, max |err|/bound {r:.2e}, 2-norm rel "
+          f"{np.linalg.norm(got - ref) / np.linalg.norm(ref):.2e}")
+    assert r < 1e-13
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < TOL
 
 
 def test_10d_L8_block16_properties(cuda):
@@ -100,16 +111,18 @@ def test_10d_L8_block16_properties(cuda):
     op.apply(X, HX)
     S = X @ HX.T
     assert (torch.max(torch.abs(S - S.T)) / torch.max(torch.abs(S))).item() < 1e-12
-    # linearity: H(2 x0 - 3 x1) = 2 H x0 - 3 H x1
+    # linearity: H(2 x0 - 3 x1) = 2 H x0 - 3 H x1 (2-norm: the max norm is
+    # limited by the conditioning of the Smolyak sum, see the parity test)
+    nrm = lambda a: torch.linalg.norm(a).item()
     Y = (2 * X[0:1] - 3 * X[1:2]).contiguous()
     HY = torch.empty_like(Y)
     op.apply(Y, HY)
     lin = 2 * HX[0:1] - 3 * HX[1:2]
-    assert (torch.max(torch.abs(HY - lin)) / torch.max(torch.abs(lin))).item() < 1e-12
+    assert nrm(HY - lin) / nrm(lin) < 1e-12
     # a single vector through the one-vector path equals its column of the block
     H3 = torch.empty_like(X[3:4])
     op.apply(X[3:4].contiguous(), H3)
-    assert (torch.max(torch.abs(H3 - HX[3:4])) / torch.max(torch.abs(HX[3:4]))).item() < 1e-12
+    assert nrm(H3 - HX[3:4]) / nrm(HX[3:4]) < 1e-12
 
 
 def test_errors_are_reported(cuda):
@@ -122,18 +135,21 @@ def test_errors_are_reported(cuda):
 
 @pytest.mark.parametrize("D,L", [(6, 6), (10, 5)])
 def test_f32_mode_accuracy(cuda, D, L):
-    """FP32 mode contractions (FP64 gather/scatter): H.psi within 1e-5
-    relative to max|H psi| of the FP64 oracle (measured: a few 1e-7; the
-    tolerance leaves room for the roundoff amplification of the Smolyak
-    combination)."""
+    """FP32 mode contractions (FP64 gather/scatter) against the FP64 oracle:
+    per element within 1e-5 of sum_t |W_t r_t| (the condition of the
+    Smolyak sum, ~170 FP32 roundoffs), and 2-norm relative error < 1e-5
+    (measured 2.4e-6 at 6D L=6)."""
     b, op, o, V = hh_pair(D, L)
     psi = np.random.default_rng(99).standard_normal((b.nb_ba, 4))
     ref = o.hpsi(psi, V, -0.5 * np.ones(D))
+    bound = o.absbound(psi, V, -0.5 * np.ones(D))
     got = op.apply_host(psi, prec=E.PREC_F32)
-    err = rel(got, ref)
-    print(f"f32 rel err {err:.2e}")
-    assert err < 1e-5
-    assert err > 1e-12   # it really ran in reduced precision
+    err2 = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    print(f"f32 max-norm rel {rel(got, ref):.2e}, 2-norm rel {err2:.2e}, "
+          f"max |err|/bound {np.max(np.abs(got - ref) / bound):.2e}")
+    assert np.max(np.abs(got - ref) / bound) < 1e-5
+    assert err2 < 1e-5
+    assert rel(got, ref) > 1e-12   # it really ran in reduced precision
 
 
 @pytest.mark.parametrize("L,nvec", [(4, 3), (10, 5)])
