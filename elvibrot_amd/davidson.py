@@ -116,7 +116,10 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
         return torch.stack(keep) if keep else None
 
     phase_prec = prec
-    switch_resi = max(conv_resi, 2e-4) if switch_resi is None else switch_resi
+    # the FP32 residual floor is ~ eps32 x (Smolyak roundoff amplification) x ||H||
+    switch_resi = max(conv_resi, 1e-3) if switch_resi is None else switch_resi
+    best_low = np.inf
+    stall = 0
     n_low = 0
     it = 0
     for it in range(max_it + 1):
@@ -152,7 +155,10 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
         if verbose:
             print(f"davidson it {it} ndim {ndim} max|r| {rn.max():.2e} max|dE| {dE.max():.2e} conv {conv.sum()}")
         Ene0 = E.copy()
-        if phase_prec != PREC_FP64 and rn.max() < switch_resi:
+        if phase_prec != PREC_FP64:
+            stall = stall + 1 if rn.max() > 0.7 * best_low else 0
+            best_low = min(best_low, rn.max())
+        if phase_prec != PREC_FP64 and (rn.max() < switch_resi or stall >= 6):
             # FP64 phase: restart on the Ritz vectors with an FP64 H.psi
             n_low = n_hpsi
             keep = min(ndim, nb_WP + block)

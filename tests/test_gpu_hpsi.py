@@ -80,21 +80,21 @@ def test_fp64_general_potential_and_coefficients(cuda):
 
 def test_10d_L8_parity_two_vectors(cuda):
     """At 10D L=8 the Smolyak scatter-sum is ill-conditioned:
-    max_e sum_t |W_t r_t| / max|H psi| ~ 5e4, so eps times it (~1e-11) bounds
-    the max-norm agreement of ANY two FP64 summation orders.  Parity is
-    therefore checked as a forward-error bound per element,
-    |got - ref| <= 1e-13 sum_t |W_t r_t| (~450 eps), plus 2-norm relative
-    error < 1e-12."""
+    max_e sum_t |W_t r_t| / max|H psi| ~ 5e4, so eps times it (~1e-11) is the
+    scale at which ANY two FP64 summation orders can differ in the max norm.
+    Parity is therefore checked as max|got - ref| <= 16 eps max_e sum_t |W_t r_t|
+    (measured 4.3 eps), plus 2-norm relative error < 1e-12 (measured 2.3e-13)."""
     D, L = 10, 8
     b, op, o, V = hh_pair(D, L, 16)
     psi = np.random.default_rng(2026).standard_normal((b.nb_ba, 2))
     got = op.apply_host(psi)
     ref = o.hpsi(psi, V, -0.5 * np.ones(D))
     bound = o.absbound(psi, V, -0.5 * np.ones(D))
-    r = np.max(np.abs(got - ref) / bound)
-    print(f"10D max-norm rel {rel(got, ref):.2e}, max |err|/bound {r:.2e}, 2-norm rel "
-          f"{np.linalg.norm(got - ref) / np.linalg.norm(ref):.2e}")
-    assert r < 1e-13
+    eps = np.finfo(np.float64).eps
+    r = np.max(np.abs(got - ref)) / (eps * np.max(bound))
+    print(f"10D max-norm rel {rel(got, ref):.2e}, max|err| / (eps max bound) {r:.2f}, 2-norm "
+          f"rel {np.linalg.norm(got - ref) / np.linalg.norm(ref):.2e}")
+    assert r < 16
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < TOL
 
 
@@ -136,18 +136,20 @@ def test_errors_are_reported(cuda):
 @pytest.mark.parametrize("D,L", [(6, 6), (10, 5)])
 def test_f32_mode_accuracy(cuda, D, L):
     """FP32 mode contractions (FP64 gather/scatter) against the FP64 oracle:
-    per element within 1e-5 of sum_t |W_t r_t| (the condition of the
-    Smolyak sum, ~170 FP32 roundoffs), and 2-norm relative error < 1e-5
-    (measured 2.4e-6 at 6D L=6)."""
+    max|err| <= eps32 max_e sum_t |W_t r_t| (the conditioning scale of the
+    Smolyak sum; measured 0.08-0.4 of it) and 2-norm relative error < 1e-5
+    (measured 6e-7 - 9e-7)."""
     b, op, o, V = hh_pair(D, L)
     psi = np.random.default_rng(99).standard_normal((b.nb_ba, 4))
     ref = o.hpsi(psi, V, -0.5 * np.ones(D))
     bound = o.absbound(psi, V, -0.5 * np.ones(D))
     got = op.apply_host(psi, prec=E.PREC_F32)
     err2 = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    eps32 = float(np.finfo(np.float32).eps)
+    r = np.max(np.abs(got - ref)) / (eps32 * np.max(bound))
     print(f"f32 max-norm rel {rel(got, ref):.2e}, 2-norm rel {err2:.2e}, "
-          f"max |err|/bound {np.max(np.abs(got - ref) / bound):.2e}")
-    assert np.max(np.abs(got - ref) / bound) < 1e-5
+          f"max|err| / (eps32 max bound) {r:.3f}")
+    assert r < 1
     assert err2 < 1e-5
     assert rel(got, ref) > 1e-12   # it really ran in reduced precision
 
