@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -1405,6 +1407,9 @@ int sg4_plan_autotune(sg4_plan* p, int32_t nvec, int32_t prec, float* best_ms) {
           float ms = 0;
           HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
           ms /= 3;
+          if (std::getenv("SG4_AUTOTUNE_VERBOSE"))
+            fprintf(stderr, "sg4 autotune nvec=%d prec=%d threads=%d lds=%d grid=%d xcd=%d: %.3f ms\n",
+                    nvec, prec, c.nthreads, c.lds_budget, c.grid, c.xcd, ms);
           if (ms < bms) { bms = ms; best = c; }
         }
   (void)hipEventDestroy(e0);
