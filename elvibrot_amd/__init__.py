@@ -215,6 +215,10 @@ class SG4Operator:
                                           C.c_void_p(out.data_ptr()), prec))
         return out
 
+    def set_grouping(self, on=True):
+        """Register-blocked mode groups (default) or one pass per mode."""
+        _capi.check(lib().sg4_plan_set_option(self._h, _capi.OPT_GROUPING, 1 if on else 0))
+
     def timing(self, enable=True):
         _capi.check(lib().sg4_plan_timing(self._h, 1 if enable else 0))
 

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "sg4_internal.h"
+#include "sg4_shapes.h"
 
 namespace sg4 {
 
@@ -64,6 +65,10 @@ struct TermDev {
   int32_t nact, pad;
   int16_t mat[MAXACT];   // MatDev index of each active mode
   int16_t kmode[MAXACT]; // index of the mode's kinetic coefficient (ck)
+  // register-blocked passes: groups of consecutive active modes
+  int32_t ngrp;          // 0: use the one-mode-per-pass path
+  int8_t gfirst[MAXACT]; // first active mode of group g
+  int8_t gshape[MAXACT]; // shape id (sg4_shapes.h)
 };
 
 struct Item {
@@ -240,16 +245,194 @@ __device__ __forceinline__ void dispatch_pass(const PassCtx<T>& c, int NI, int N
 }
 
 // ---------------------------------------------------------------------------
-// Type-1 operator: one workgroup per work item.  T is the type of the mode
-// contractions (double: parity mode; float: the FP32 mode); gather and
-// scatter stay in FP64.
+// Register-blocked group passes.  A slab = all grid values of a group of up to
+// three consecutive active modes (sizes N0, N1, N2) for fixed indices of the
+// other modes and one vector; one thread holds a slab in registers and applies
+// every 1-D matrix of the group before writing it back, so a term costs one
+// pass per group instead of one per mode.
+//   KIND_CHAIN: x <- M2 x3 (M1 x2 (M0 x1 x))          (basis<->grid, in place)
+//   KIND_GRID : y <- [V x | y] + sum_i c_i (D2_i xi x)  (potential + kinetic)
+// Slab element J (first index fastest) of slab f lives at base + AV*J with
+// base = (f mod AV) + AV*P*(f div AV), P = N0 N1 N2, AV = A*nv.
 // ---------------------------------------------------------------------------
+enum { KIND_CHAIN = 0, KIND_GRID = 1 };
+
 template <class T>
 struct KMats {
   const T* mats;
   const T* V;
   const T* ck;
 };
+
+
+template <class T>
+struct GCtx {
+  const T* M[3];   // row-major n x n matrices of the group's modes
+  T c[3];          // kinetic coefficients (grid kind)
+  const T* in;
+  T* out;
+  int AV;
+  float invAV;
+  int A;
+  int nfib;
+  int lgnv;
+  int first;       // grid kind: start from V x instead of the accumulator
+  const T* __restrict__ V;
+  T vshift;
+};
+
+// y = M x along one mode of the slab (stride S, N entries), in place
+template <class T, int N, int S, int P>
+__device__ __forceinline__ void slab_mode(T (&x)[P], cptr<T> M) {
+#pragma unroll
+  for (int o = 0; o < P / N; ++o) {
+    const int base = (o % S) + (o / S) * S * N;
+    T y[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      T acc = 0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) acc = fmat(M[i * N + j], x[base + S * j], acc);
+      y[i] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[base + S * i] = y[i];
+  }
+}
+
+// acc += c (M x) along one mode of the slab
+template <class T, int N, int S, int P>
+__device__ __forceinline__ void slab_mode_acc(const T (&x)[P], T (&acc)[P], cptr<T> M, T c) {
+#pragma unroll
+  for (int o = 0; o < P / N; ++o) {
+    const int base = (o % S) + (o / S) * S * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      T y = 0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) y = fmat(M[i * N + j], x[base + S * j], y);
+      acc[base + S * i] = fmat(c, y, acc[base + S * i]);
+    }
+  }
+}
+
+template <class T, int N0, int N1, int N2, int KIND>
+__device__ __forceinline__ void group_pass(const GCtx<T>& c) {
+  constexpr int P = N0 * N1 * N2;
+  for (int f = threadIdx.x; f < c.nfib; f += blockDim.x) {
+    const int b = fdiv(f, c.AV, c.invAV);
+    const int av = f - b * c.AV;
+    const int base = av + c.AV * P * b;
+    T x[P];
+#pragma unroll
+    for (int J = 0; J < P; ++J) x[J] = c.in[base + c.AV * J];
+    cptr<T> M0 = (cptr<T>)uniform_ptr(c.M[0]);
+    cptr<T> M1 = (cptr<T>)uniform_ptr(c.M[1]);
+    cptr<T> M2 = (cptr<T>)uniform_ptr(c.M[2]);
+    asm volatile("" : "+s"(M0), "+s"(M1), "+s"(M2));
+    if constexpr (KIND == KIND_CHAIN) {
+      slab_mode<T, N0, 1, P>(x, M0);
+      if constexpr (N1 > 1) slab_mode<T, N1, N0, P>(x, M1);
+      if constexpr (N2 > 1) slab_mode<T, N2, N0 * N1, P>(x, M2);
+#pragma unroll
+      for (int J = 0; J < P; ++J) c.out[base + c.AV * J] = x[J];
+    } else {
+      T acc[P];
+      if (c.first) {
+        const int e0 = (av >> c.lgnv) + c.A * P * b;
+#pragma unroll
+        for (int J = 0; J < P; ++J) acc[J] = (c.V[e0 + c.A * J] + c.vshift) * x[J];
+      } else {
+#pragma unroll
+        for (int J = 0; J < P; ++J) acc[J] = c.out[base + c.AV * J];
+      }
+      slab_mode_acc<T, N0, 1, P>(x, acc, M0, c.c[0]);
+      if constexpr (N1 > 1) slab_mode_acc<T, N1, N0, P>(x, acc, M1, c.c[1]);
+      if constexpr (N2 > 1) slab_mode_acc<T, N2, N0 * N1, P>(x, acc, M2, c.c[2]);
+#pragma unroll
+      for (int J = 0; J < P; ++J) c.out[base + c.AV * J] = acc[J];
+    }
+  }
+}
+
+template <class T, int KIND>
+__device__ __forceinline__ void dispatch_group(const GCtx<T>& c, int shape) {
+  switch (shape) {
+#define SG4_GCASE(id, a, b, d) case id: group_pass<T, a, b, d, KIND>(c); return;
+    SG4_SHAPES(SG4_GCASE)
+#undef SG4_GCASE
+    default: return;
+  }
+}
+
+// One term with every active mode square (nq = nb), through group passes:
+// gather -> basis->grid groups (in place) -> grid groups (into the second
+// buffer) -> grid->basis groups (in place) -> weighted scatter-add.
+template <class T>
+__device__ __forceinline__ void term_t1_grouped(const KArgs& a, const KMats<T>& km,
+                                                const Item& it, T* lds) {
+  const TermDev* Td = &a.terms[it.term];
+  const int lg = it.lgnv, nv = 1 << lg;
+  const int32_t* map = a.map + Td->boff;
+  const int N = Td->nq;
+  const int NE = N * nv;
+  T* X = lds;
+  T* Y = lds + NE;
+  for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+    const int m = map[e >> lg];
+    X[e] = (m >= 0) ? (T)a.psi[(int64_t)m * a.nvec + it.v0 + (e & (nv - 1))] : (T)0;
+  }
+  __syncthreads();
+  GCtx<T> c;
+  c.lgnv = lg;
+  c.V = km.V + Td->qoff;
+  c.vshift = (T)Td->vshift;
+  for (int stage = 0; stage < 3; ++stage) {
+    int A = 1;
+    for (int g = 0; g < Td->ngrp; ++g) {
+      const int p0 = Td->gfirst[g];
+      const int p1 = (g + 1 < Td->ngrp) ? Td->gfirst[g + 1] : Td->nact;
+      int P = 1;
+      for (int q = 0; q < 3; ++q) {
+        const int pm = p0 + q;
+        if (pm < p1) {
+          const MatDev* md = &a.md[Td->mat[pm]];
+          P *= md->nq;
+          c.M[q] = km.mats + (stage == 0 ? md->B : stage == 1 ? md->D2 : md->BtW);
+          c.c[q] = km.ck[Td->kmode[pm]];
+        } else {
+          c.M[q] = c.M[0];
+          c.c[q] = 0;
+        }
+      }
+      c.AV = A * nv;
+      c.invAV = 1.0f / (float)c.AV;
+      c.A = A;
+      c.nfib = NE / P;
+      if (stage == 1) {
+        c.in = X; c.out = Y; c.first = (g == 0);
+        dispatch_group<T, KIND_GRID>(c, Td->gshape[g]);
+      } else {
+        T* buf = (stage == 0) ? X : Y;
+        c.in = buf; c.out = buf; c.first = 0;
+        dispatch_group<T, KIND_CHAIN>(c, Td->gshape[g]);
+      }
+      __syncthreads();
+      A *= P;
+    }
+  }
+  const double w = Td->wgt;
+  for (int e = threadIdx.x; e < NE; e += blockDim.x) {
+    const int m = map[e >> lg];
+    if (m >= 0) atomicAdd(&a.hpsi[(int64_t)m * a.nvec + it.v0 + (e & (nv - 1))], w * (double)Y[e]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Type-1 operator: one workgroup per work item.  T is the type of the mode
+// contractions (double: parity mode; float: the FP32 mode); gather and
+// scatter stay in FP64.
+// ---------------------------------------------------------------------------
 
 template <class T, bool BIG>
 __device__ __forceinline__ void term_t1(const KArgs& a, const KMats<T>& km, const Item& it,
@@ -352,6 +535,7 @@ __global__ __launch_bounds__(NT) void k_hpsi_t1_f64(KArgs a, KMats<double> km) {
     const Item it = a.items[w];
     if (it.term < 0) continue;
     if (it.big) term_t1<double, true>(a, km, it, lds);
+    else if (a.terms[it.term].ngrp > 0) term_t1_grouped<double>(a, km, it, lds);
     else term_t1<double, false>(a, km, it, lds);
   }
 }
@@ -364,6 +548,7 @@ __global__ __launch_bounds__(NT) void k_hpsi_t1_f32(KArgs a, KMats<float> km) {
     const Item it = a.items[w];
     if (it.term < 0) continue;
     if (it.big) term_t1<float, true>(a, km, it, lds);
+    else if (a.terms[it.term].ngrp > 0) term_t1_grouped<float>(a, km, it, lds);
     else term_t1<float, false>(a, km, it, lds);
   }
 }
@@ -719,6 +904,7 @@ struct sg4_plan {
   std::vector<GraphEntry> graphs;
   double flops_per_vec = 0, bytes_fixed = 0;
   int64_t last_items = -1;   // work items of the last apply
+  bool grouping = true;      // register-blocked group passes for square terms
   // optional kernel timing: HIP events around the H.psi launch of each apply
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -732,6 +918,41 @@ struct sg4_plan {
 };
 
 namespace sg4 {
+
+// Greedy grouping of consecutive active modes into the shapes of
+// sg4_shapes.h (triples, then pairs, then singles); 0 if some mode is not
+// square or has no single-mode shape (the term then uses one pass per mode).
+static int make_groups(const sg4_plan* P, TermDev& T) {
+  static const int shapes[][3] = {
+#define SG4_ROW(id, a, b, d) {a, b, d},
+      SG4_SHAPES(SG4_ROW)
+#undef SG4_ROW
+  };
+  auto find = [&](int a, int b, int d) {
+    for (int i = 0; i < SG4_NSHAPES; ++i)
+      if (shapes[i][0] == a && shapes[i][1] == b && shapes[i][2] == d) return i;
+    return -1;
+  };
+  int n[MAXACT];
+  for (int p = 0; p < T.nact; ++p) {
+    const MatDev& m = P->h_md[T.mat[p]];
+    if (m.nq != m.nb) return 0;
+    n[p] = m.nq;
+  }
+  int g = 0, p = 0;
+  while (p < T.nact) {
+    int id = -1, len = 0;
+    if (p + 2 < T.nact && (id = find(n[p], n[p + 1], n[p + 2])) >= 0) len = 3;
+    else if (p + 1 < T.nact && (id = find(n[p], n[p + 1], 1)) >= 0) len = 2;
+    else if ((id = find(n[p], 1, 1)) >= 0) len = 1;
+    else return 0;
+    T.gfirst[g] = (int8_t)p;
+    T.gshape[g] = (int8_t)id;
+    ++g;
+    p += len;
+  }
+  return g;
+}
 
 static int build_terms(sg4_plan* P) {
   P->h_terms.resize(P->nterms);
@@ -769,6 +990,7 @@ static int build_terms(sg4_plan* P) {
       T.nact = nact;
       T.wgt = w;
       T.vshift = vs;
+      T.ngrp = (P->grouping && P->type_op == 1 && nact > 0) ? make_groups(P, T) : 0;
       P->h_terms[t] = T;
     }
   }
@@ -1469,6 +1691,15 @@ int sg4_apply_graph(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, 
   HIP_TRY(hipStreamWaitEvent(user, ev, 0));
   (void)hipEventDestroy(ev);
   return SG4_OK;
+}
+
+int sg4_plan_set_option(sg4_plan* p, int32_t option, int32_t value) {
+  if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
+  if (option != SG4_OPT_GROUPING) { set_error("unknown option"); return SG4_ERR_ARG; }
+  p->grouping = value != 0;
+  if (p->type_op == 0) return SG4_OK;
+  p->items.clear();
+  return build_terms(p);
 }
 
 int sg4_plan_timing(sg4_plan* p, int32_t enable) {

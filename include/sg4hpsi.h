@@ -180,6 +180,12 @@ int sg4_plan_config(const sg4_plan *p, int32_t nvec, int32_t prec, int32_t *cfg4
  * stream and the plan's stream waits for them. */
 int sg4_apply_graph(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);
 
+/* Plan options.  SG4_OPT_GROUPING (default 1): terms whose active modes are
+ * all square go through register-blocked passes over groups of up to three
+ * modes; 0 forces one pass per mode (both give the same operator). */
+#define SG4_OPT_GROUPING 1
+int sg4_plan_set_option(sg4_plan *p, int32_t option, int32_t value);
+
 /* Statistics of the last apply: launches, items, and the algorithmic flop and
  * byte counts per block apply used for roofline reporting. */
 int sg4_plan_stats(const sg4_plan *p, int32_t nvec, double *flops, double *bytes,

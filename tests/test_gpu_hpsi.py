@@ -201,3 +201,19 @@ def test_graph_replays_match_eager(cuda):
         op.apply_rows(X * (r + 1), Z)          # eager work between replays
         torch.cuda.synchronize()
         assert torch.max(torch.abs(Z - (r + 1) * ref)).item() <= 1e-12 * torch.max(torch.abs(Z)).item()
+
+
+@pytest.mark.parametrize("D,L,prec", [(6, 6, E.PREC_FP64), (4, 6, E.PREC_FP64), (6, 6, E.PREC_F32)])
+def test_grouped_and_per_mode_passes_agree(cuda, D, L, prec):
+    """Register-blocked mode groups (default) and one pass per mode give the
+    same operator (FP64: to rounding, 1e-13 relative; FP32: 1e-5)."""
+    b, op, o, V = hh_pair(D, L)
+    psi = np.random.default_rng(21).standard_normal((b.nb_ba, 5))
+    g = op.apply_host(psi, prec)
+    op.set_grouping(False)
+    u = op.apply_host(psi, prec)
+    op.set_grouping(True)
+    tol = 1e-13 if prec == E.PREC_FP64 else 1e-5
+    assert rel(g, u) < tol
+    if prec == E.PREC_FP64:
+        assert rel(g, o.hpsi(psi, V, -0.5 * np.ones(D))) < TOL
