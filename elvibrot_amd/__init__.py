@@ -216,8 +216,16 @@ class SG4Operator:
         return out
 
     def set_grouping(self, on=True):
-        """Register-blocked mode groups (default) or one pass per mode."""
+        """Register-blocked mode groups and, where it applies, the split path
+        (kinetic part as one sparse matrix; default), or one pass per mode with
+        the kinetic part on the grid (the reference's structure)."""
         _capi.check(lib().sg4_plan_set_option(self._h, _capi.OPT_GROUPING, 1 if on else 0))
+
+    def split_info(self):
+        """(split path in use, stored entries of the kinetic matrix)"""
+        sp, nnz = C.c_int32(), C.c_int64()
+        _capi.check(lib().sg4_plan_split_info(self._h, C.byref(sp), C.byref(nnz)))
+        return bool(sp.value), nnz.value
 
     def timing(self, enable=True):
         _capi.check(lib().sg4_plan_timing(self._h, 1 if enable else 0))

@@ -275,10 +275,7 @@ struct GCtx {
   float invAV;
   int A;
   int nfib;
-  int lgnv;
-  int first;       // grid kind: start from V x instead of the accumulator
-  const T* __restrict__ V;
-  T vshift;
+  int first;       // grid kind: out holds V + vshift (staged by the gather); start from V x
 };
 
 // y = M x along one mode of the slab (stride S, N entries), in place
@@ -339,9 +336,8 @@ __device__ __forceinline__ void group_pass(const GCtx<T>& c) {
     } else {
       T acc[P];
       if (c.first) {
-        const int e0 = (av >> c.lgnv) + c.A * P * b;
 #pragma unroll
-        for (int J = 0; J < P; ++J) acc[J] = (c.V[e0 + c.A * J] + c.vshift) * x[J];
+        for (int J = 0; J < P; ++J) acc[J] = c.out[base + c.AV * J] * x[J];
       } else {
 #pragma unroll
         for (int J = 0; J < P; ++J) acc[J] = c.out[base + c.AV * J];
@@ -365,28 +361,53 @@ __device__ __forceinline__ void dispatch_group(const GCtx<T>& c, int shape) {
   }
 }
 
+// Gather and scatter walk the item in batches of GK elements per thread so
+// that the map loads of a batch, then its psi loads (or atomics), are all in
+// flight together instead of one dependent load pair per element.
+constexpr int GK = 8;
+
 // One term with every active mode square (nq = nb), through group passes:
-// gather -> basis->grid groups (in place) -> grid groups (into the second
-// buffer) -> grid->basis groups (in place) -> weighted scatter-add.
+// gather (psi rows into X, V + vshift into Y) -> basis->grid groups (in
+// place on X) -> grid groups (X -> Y, the first one multiplying by the staged
+// potential) -> grid->basis groups (in place on Y) -> weighted scatter-add.
 template <class T>
 __device__ __forceinline__ void term_t1_grouped(const KArgs& a, const KMats<T>& km,
                                                 const Item& it, T* lds) {
   const TermDev* Td = &a.terms[it.term];
   const int lg = it.lgnv, nv = 1 << lg;
   const int32_t* map = a.map + Td->boff;
+  const T* V = km.V + Td->qoff;
+  const T vshift = (T)Td->vshift;
   const int N = Td->nq;
   const int NE = N * nv;
+  const int nt = blockDim.x;
   T* X = lds;
   T* Y = lds + NE;
-  for (int e = threadIdx.x; e < NE; e += blockDim.x) {
-    const int m = map[e >> lg];
-    X[e] = (m >= 0) ? (T)a.psi[(int64_t)m * a.nvec + it.v0 + (e & (nv - 1))] : (T)0;
+  for (int e0 = threadIdx.x; e0 < NE; e0 += GK * nt) {
+    int m[GK];
+    T v[GK], x[GK];
+#pragma unroll
+    for (int k = 0; k < GK; ++k) {
+      const int e = e0 + k * nt;
+      m[k] = (e < NE) ? map[e >> lg] : -1;
+      v[k] = (e < NE) ? V[e >> lg] : (T)0;
+    }
+#pragma unroll
+    for (int k = 0; k < GK; ++k) {
+      const int e = e0 + k * nt;
+      x[k] = (m[k] >= 0) ? (T)a.psi[(int64_t)m[k] * a.nvec + it.v0 + (e & (nv - 1))] : (T)0;
+    }
+#pragma unroll
+    for (int k = 0; k < GK; ++k) {
+      const int e = e0 + k * nt;
+      if (e < NE) {
+        X[e] = x[k];
+        Y[e] = v[k] + vshift;
+      }
+    }
   }
   __syncthreads();
   GCtx<T> c;
-  c.lgnv = lg;
-  c.V = km.V + Td->qoff;
-  c.vshift = (T)Td->vshift;
   for (int stage = 0; stage < 3; ++stage) {
     int A = 1;
     for (int g = 0; g < Td->ngrp; ++g) {
@@ -422,9 +443,248 @@ __device__ __forceinline__ void term_t1_grouped(const KArgs& a, const KMats<T>& 
     }
   }
   const double w = Td->wgt;
-  for (int e = threadIdx.x; e < NE; e += blockDim.x) {
-    const int m = map[e >> lg];
-    if (m >= 0) atomicAdd(&a.hpsi[(int64_t)m * a.nvec + it.v0 + (e & (nv - 1))], w * (double)Y[e]);
+  for (int e0 = threadIdx.x; e0 < NE; e0 += GK * nt) {
+    int m[GK];
+#pragma unroll
+    for (int k = 0; k < GK; ++k) {
+      const int e = e0 + k * nt;
+      m[k] = (e < NE) ? map[e >> lg] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < GK; ++k) {
+      const int e = e0 + k * nt;
+      if (m[k] >= 0)
+        atomicAdd(&a.hpsi[(int64_t)m[k] * a.nvec + it.v0 + (e & (nv - 1))], w * (double)Y[e]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Split type-1 path.  When the quadrature of every 1-D basis integrates the
+// products of two basis functions exactly (B^T W B = I, checked when the plan
+// is built), the kinetic part of a term needs no grid: along mode k it is
+// c_k T_k(l_k) with T_k(l) = BtW D2 B (nb x nb), the identity in the other
+// modes.  Summed over the Smolyak terms with the combination weights, the
+// kinetic part of H becomes one sparse matrix K between packed rows that
+// differ in one mode index (built on the host, applied by k_kin_rows), and a
+// term only needs  W_t P_t^T B^T W V_t B P_t :  basis->grid products, the
+// potential, grid->basis products.  The grid->basis groups run in reverse
+// order so that the last basis->grid group, the potential and the first
+// grid->basis group are one register pass.
+// LDS: X (N nv values of the item) followed by Vs (the N potential values).
+// ---------------------------------------------------------------------------
+template <class T>
+struct SCtx {
+  const T* M[3];   // basis->grid (or, not fused, the pass's) matrices of the group
+  const T* Mt[3];  // fused pass: grid->basis matrices
+  T* X;
+  const T* Vs;
+  int AV;
+  float invAV;
+  int nfib;
+  int lgnv;
+};
+
+template <class T, int N0, int N1, int N2, bool FUSED>
+__device__ __forceinline__ void split_pass(const SCtx<T>& c) {
+  constexpr int P = N0 * N1 * N2;
+  for (int f = threadIdx.x; f < c.nfib; f += blockDim.x) {
+    const int b = fdiv(f, c.AV, c.invAV);
+    const int av = f - b * c.AV;
+    const int base = av + c.AV * P * b;
+    T x[P];
+#pragma unroll
+    for (int J = 0; J < P; ++J) x[J] = c.X[base + c.AV * J];
+    {
+      cptr<T> M0 = (cptr<T>)uniform_ptr(c.M[0]);
+      cptr<T> M1 = (cptr<T>)uniform_ptr(c.M[1]);
+      cptr<T> M2 = (cptr<T>)uniform_ptr(c.M[2]);
+      asm volatile("" : "+s"(M0), "+s"(M1), "+s"(M2));
+      slab_mode<T, N0, 1, P>(x, M0);
+      if constexpr (N1 > 1) slab_mode<T, N1, N0, P>(x, M1);
+      if constexpr (N2 > 1) slab_mode<T, N2, N0 * N1, P>(x, M2);
+    }
+    if constexpr (FUSED) {
+#pragma unroll
+      for (int J = 0; J < P; ++J) x[J] *= c.Vs[(base + c.AV * J) >> c.lgnv];
+      cptr<T> Q0 = (cptr<T>)uniform_ptr(c.Mt[0]);
+      cptr<T> Q1 = (cptr<T>)uniform_ptr(c.Mt[1]);
+      cptr<T> Q2 = (cptr<T>)uniform_ptr(c.Mt[2]);
+      asm volatile("" : "+s"(Q0), "+s"(Q1), "+s"(Q2));
+      slab_mode<T, N0, 1, P>(x, Q0);
+      if constexpr (N1 > 1) slab_mode<T, N1, N0, P>(x, Q1);
+      if constexpr (N2 > 1) slab_mode<T, N2, N0 * N1, P>(x, Q2);
+    }
+#pragma unroll
+    for (int J = 0; J < P; ++J) c.X[base + c.AV * J] = x[J];
+  }
+}
+
+template <class T, bool FUSED>
+__device__ __forceinline__ void dispatch_split(const SCtx<T>& c, int shape) {
+  switch (shape) {
+#define SG4_SCASE(id, a, b, d) case id: split_pass<T, a, b, d, FUSED>(c); return;
+    SG4_SHAPES(SG4_SCASE)
+#undef SG4_SCASE
+    default: return;
+  }
+}
+
+// Potential part of one term for one chunk of vectors (every active mode
+// square and grouped): gather -> basis->grid groups 0..G-2 -> fused group
+// G-1 -> grid->basis groups G-2..0 -> weighted scatter-add.
+template <class T>
+__device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km, const Item& it,
+                                              T* lds) {
+  const TermDev* Td = &a.terms[it.term];
+  const int lg = it.lgnv, nv = 1 << lg;
+  const int32_t* map = a.map + Td->boff;
+  const T* V = km.V + Td->qoff;
+  const double w = Td->wgt;
+  if (Td->nact == 0) {  // the one-point term
+    if ((int)threadIdx.x < nv) {
+      const int m = map[0];
+      if (m >= 0) {
+        const int64_t r = (int64_t)m * a.nvec + it.v0 + threadIdx.x;
+        atomicAdd(&a.hpsi[r], w * (double)(V[0] * (T)a.psi[r]));
+      }
+    }
+    return;
+  }
+  const int N = Td->nq;
+  const int NE = N * nv;
+  const int nt = blockDim.x;
+  T* X = lds;
+  T* Vs = lds + NE;
+  for (int e0 = threadIdx.x; e0 < NE; e0 += GK * nt) {
+    int m[GK];
+    T v[GK], x[GK];
+#pragma unroll
+    for (int k = 0; k < GK; ++k) {
+      const int e = e0 + k * nt;
+      m[k] = (e < NE) ? map[e >> lg] : -1;
+      v[k] = (e < NE) ? V[e >> lg] : (T)0;
+    }
+#pragma unroll
+    for (int k = 0; k < GK; ++k) {
+      const int e = e0 + k * nt;
+      x[k] = (m[k] >= 0) ? (T)a.psi[(int64_t)m[k] * a.nvec + it.v0 + (e & (nv - 1))] : (T)0;
+    }
+#pragma unroll
+    for (int k = 0; k < GK; ++k) {
+      const int e = e0 + k * nt;
+      if (e < NE) {
+        X[e] = x[k];
+        if ((e & (nv - 1)) == 0) Vs[e >> lg] = v[k];
+      }
+    }
+  }
+  __syncthreads();
+  const int G = Td->ngrp;
+  SCtx<T> c;
+  c.X = X;
+  c.Vs = Vs;
+  c.lgnv = lg;
+  int Ag[MAXACT];
+  {
+    int A = 1;
+    for (int g = 0; g < G; ++g) {
+      Ag[g] = A;
+      const int p1 = (g + 1 < G) ? Td->gfirst[g + 1] : Td->nact;
+      for (int pm = Td->gfirst[g]; pm < p1; ++pm) A *= a.md[Td->mat[pm]].nq;
+    }
+  }
+  auto setup = [&](int g, bool fused, bool btw) {
+    const int p0 = Td->gfirst[g];
+    const int p1 = (g + 1 < G) ? Td->gfirst[g + 1] : Td->nact;
+    int P = 1;
+    for (int q = 0; q < 3; ++q) {
+      const int pm = p0 + q;
+      if (pm < p1) {
+        const MatDev* md = &a.md[Td->mat[pm]];
+        P *= md->nq;
+        c.M[q] = km.mats + (btw ? md->BtW : md->B);
+        c.Mt[q] = km.mats + md->BtW;
+      } else {
+        c.M[q] = c.M[0];
+        c.Mt[q] = c.Mt[0];
+      }
+    }
+    (void)fused;
+    c.AV = Ag[g] * nv;
+    c.invAV = 1.0f / (float)c.AV;
+    c.nfib = NE / P;
+  };
+  for (int g = 0; g + 1 < G; ++g) {
+    setup(g, false, false);
+    dispatch_split<T, false>(c, Td->gshape[g]);
+    __syncthreads();
+  }
+  setup(G - 1, true, false);
+  dispatch_split<T, true>(c, Td->gshape[G - 1]);
+  __syncthreads();
+  for (int g = G - 2; g >= 0; --g) {
+    setup(g, false, true);
+    dispatch_split<T, false>(c, Td->gshape[g]);
+    __syncthreads();
+  }
+  for (int e0 = threadIdx.x; e0 < NE; e0 += GK * nt) {
+    int m[GK];
+#pragma unroll
+    for (int k = 0; k < GK; ++k) {
+      const int e = e0 + k * nt;
+      m[k] = (e < NE) ? map[e >> lg] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < GK; ++k) {
+      const int e = e0 + k * nt;
+      if (m[k] >= 0)
+        atomicAdd(&a.hpsi[(int64_t)m[k] * a.nvec + it.v0 + (e & (nv - 1))], w * (double)X[e]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_hpsi_t1s_f64(KArgs a, KMats<double> km) {
+  extern __shared__ double lds[];
+  for (int w = blockIdx.x; w < a.nitems; w += gridDim.x) {
+    if (w != (int)blockIdx.x) __syncthreads();   // LDS reuse between items
+    const Item it = a.items[w];
+    if (it.term < 0) continue;
+    term_t1_split<double>(a, km, it, lds);
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_hpsi_t1s_f32(KArgs a, KMats<float> km) {
+  extern __shared__ double lds_d[];
+  float* lds = (float*)lds_d;
+  for (int w = blockIdx.x; w < a.nitems; w += gridDim.x) {
+    if (w != (int)blockIdx.x) __syncthreads();
+    const Item it = a.items[w];
+    if (it.term < 0) continue;
+    term_t1_split<float>(a, km, it, lds);
+  }
+}
+
+// Kinetic part of the split path: hpsi(r, v) = sum_e K(r, col_e) psi(col_e, v)
+// (CSR rows, the diagonal first); one thread per (row, vector), plain stores,
+// so it also initialises the result that the term kernel then adds to.
+__global__ void k_kin_rows(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                           const double* __restrict__ val, const double* __restrict__ psi,
+                           double* __restrict__ hpsi, int64_t nrows, int nvec) {
+  const int64_t n = nrows * nvec;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = idx / nvec;
+    const int v = (int)(idx - r * nvec);
+    const int64_t e1 = ptr[r + 1];
+    double s0 = 0.0, s1 = 0.0;
+    int64_t e = ptr[r];
+    for (; e + 1 < e1; e += 2) {
+      s0 = fma(val[e], psi[(int64_t)col[e] * nvec + v], s0);
+      s1 = fma(val[e + 1], psi[(int64_t)col[e + 1] * nvec + v], s1);
+    }
+    if (e < e1) s0 = fma(val[e], psi[(int64_t)col[e] * nvec + v], s0);
+    hpsi[idx] = s0 + s1;
   }
 }
 
@@ -857,6 +1117,10 @@ struct ItemPlan {
   DevBuf items, scratch;
   int64_t nitems = 0, nbig = 0, slot_size = 0;
   int lds = 0;
+  // split path: items whose LDS need exceeds the budget, in a launch of their own
+  DevBuf items2;
+  int64_t nitems2 = 0;
+  int lds2 = 0;
 };
 
 // Launch configuration of the H.psi kernel; chosen by sg4_plan_autotune.
@@ -905,6 +1169,13 @@ struct sg4_plan {
   double flops_per_vec = 0, bytes_fixed = 0;
   int64_t last_items = -1;   // work items of the last apply
   bool grouping = true;      // register-blocked group passes for square terms
+  // split type-1 path: kinetic sparse matrix K (CSR over packed rows) and the
+  // host copies of the map and matrices it is assembled from
+  bool split_ok = false;
+  int64_t kin_nnz = 0;
+  std::vector<int32_t> h_map;
+  std::vector<double> h_mats;
+  sg4::DevBuf d_kptr, d_kcol, d_kval;
   // optional kernel timing: HIP events around the H.psi launch of each apply
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -954,6 +1225,190 @@ static int make_groups(const sg4_plan* P, TermDev& T) {
   return g;
 }
 
+static double binom(int n, int k) {
+  if (k < 0 || k > n) return 0.0;
+  double r = 1.0;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+
+// Kinetic matrix of the split type-1 path (see the device comment above
+// split_pass).  For a packed row r with mode indices i_j, let lambda_j be the
+// smallest level whose basis holds i_j and Lambda_k = sum_{j != k} lambda_j.
+// The terms holding r and a row r' that differs in mode k only are those with
+// l_j >= lambda_j (j != k) and l_k >= the levels of i_k and i'_k; their
+// combination weights W(|l|) = (-1)^(L-|l|) C(D-1, L-|l|) sum, for fixed l_k,
+// to sum_e C(e+D-2, D-2) W(l_k + Lambda_k + e), the coefficient of z^u in
+// (1-z)^-(D-1) (1-z)^(D-1) = 1 with u = L - l_k - Lambda_k, i.e. to 1 for
+// l_k = L - Lambda_k and 0 otherwise.  Hence
+//   K[r][r'] = c_k T_k(L - Lambda_k)[i_k][i'_k],  T_k(m) = BtW D2 B at level m,
+// read off the term l* = (lambda_1 .. L - Lambda_k .. lambda_D) (|l*| = L),
+// whose map holds both rows.  Every (row, mode) pair is visited once: term l*
+// with |l*| = L, mode k, mode indices j != k in the top level band of l*_j.
+// Entries of T_k(m) below 1e-12 max|T_k(m)| are quadrature rounding of
+// integrals that vanish exactly (odd parity, and |i - j| > 2 for HO bases;
+// they are ~1e-16 relative, the others >= 1e-2) and are not stored.
+// The split path is enabled only when every 1-D basis has B^T W B = I to
+// 1e-11, the terms are exactly the Smolyak set {L-D+1 <= |l| <= L} with
+// these weights, and every term is square and grouped.
+static int build_kinetic(sg4_plan* P) {
+  P->split_ok = false;
+  if (P->type_op != 1 || !P->grouping) return SG4_OK;
+  for (const TermDev& T : P->h_terms) {
+    if (T.nact > 0 && T.ngrp == 0) return SG4_OK;
+    if (16.0 * T.nq > (double)LDS_MAX) return SG4_OK;   // FP64: one vector + potential
+  }
+  const size_t nmd = P->h_md.size();
+  std::vector<int64_t> toff(nmd);
+  std::vector<double> tm, tcut(nmd);
+  for (size_t i = 0; i < nmd; ++i) {
+    const MatDev& m = P->h_md[i];
+    if (m.nq != m.nb) return SG4_OK;
+    const int n = m.nq;
+    const double* B = &P->h_mats[m.B];     // n x n, row-major
+    const double* Bt = &P->h_mats[m.BtW];
+    const double* D2 = &P->h_mats[m.D2];
+    std::vector<long double> DB((size_t)n * n);
+    for (int a = 0; a < n; ++a)
+      for (int j = 0; j < n; ++j) {
+        long double sI = 0, sD = 0;
+        for (int b = 0; b < n; ++b) {
+          sI += (long double)Bt[a * n + b] * B[b * n + j];
+          sD += (long double)D2[a * n + b] * B[b * n + j];
+        }
+        if (std::fabs((double)sI - (a == j ? 1.0 : 0.0)) > 1e-11) return SG4_OK;
+        DB[(size_t)a * n + j] = sD;
+      }
+    toff[i] = (int64_t)tm.size();
+    double tmax = 0.0;
+    for (int a = 0; a < n; ++a)
+      for (int j = 0; j < n; ++j) {
+        long double t = 0;
+        for (int b = 0; b < n; ++b) t += (long double)Bt[a * n + b] * DB[(size_t)b * n + j];
+        tm.push_back((double)t);
+        tmax = std::max(tmax, std::fabs((double)t));
+      }
+    tcut[i] = 1e-12 * tmax;
+  }
+  // Smolyak structure of every instance
+  std::vector<int> Linst(P->inst.size());
+  for (size_t ii = 0; ii < P->inst.size(); ++ii) {
+    const sg4_plan::Inst& I = P->inst[ii];
+    const int D = I.D;
+    int L = 0;
+    for (int64_t tl = 0; tl < I.nterms; ++tl) {
+      int sl = 0;
+      for (int k = 0; k < D; ++k) sl += P->h_levels[I.lev0 + tl * D + k];
+      L = std::max(L, sl);
+    }
+    if (L > I.Lmax) return SG4_OK;
+    double cnt = 0;
+    for (int sl = std::max(0, L - D + 1); sl <= L; ++sl) cnt += binom(sl + D - 1, D - 1);
+    if (cnt != (double)I.nterms) return SG4_OK;
+    for (int64_t tl = 0; tl < I.nterms; ++tl) {
+      int sl = 0;
+      for (int k = 0; k < D; ++k) sl += P->h_levels[I.lev0 + tl * D + k];
+      const double wexp = (((L - sl) & 1) ? -1.0 : 1.0) * binom(D - 1, L - sl);
+      if (std::fabs(P->h_weights[I.term0 + tl] - wexp) > 1e-12 * std::fabs(wexp)) return SG4_OK;
+    }
+    Linst[ii] = L;
+  }
+  // two sweeps over (row, mode) pairs: count, then fill
+  const int64_t nrows = P->nb_ba;
+  std::vector<int64_t> ptr(nrows + 1, 0);
+  std::vector<int32_t> col;
+  std::vector<double> val;
+  std::vector<double> diag;
+  for (int sweep = 0; sweep < 2; ++sweep) {
+    std::vector<int64_t> pos;
+    if (sweep == 1) {
+      for (int64_t r = 0; r < nrows; ++r) ptr[r + 1] += ptr[r] + 1;   // + the diagonal
+      col.assign(ptr[nrows], 0);
+      val.assign(ptr[nrows], 0.0);
+      diag.assign(nrows, 0.0);
+      pos.resize(nrows);
+      for (int64_t r = 0; r < nrows; ++r) pos[r] = ptr[r] + 1;
+    }
+    for (size_t ii = 0; ii < P->inst.size(); ++ii) {
+      const sg4_plan::Inst& I = P->inst[ii];
+      const int D = I.D, L1 = I.Lmax + 1, L = Linst[ii];
+      auto nbk = [&](int k, int l) { return l < 0 ? 0 : P->h_nb[I.md0 + k * L1 + l]; };
+      int lv[SG4_MAX_D], lo[SG4_MAX_D], hi[SG4_MAX_D], dig[SG4_MAX_D];
+      int64_t stride[SG4_MAX_D + 1];
+      for (int64_t tl = 0; tl < I.nterms; ++tl) {
+        int sl = 0;
+        for (int k = 0; k < D; ++k) sl += (lv[k] = P->h_levels[I.lev0 + tl * D + k]);
+        if (sl != L) continue;
+        const int64_t t = I.term0 + tl;
+        const int32_t* mp = &P->h_map[P->h_boff[t]];
+        stride[0] = 1;
+        for (int k = 0; k < D; ++k) stride[k + 1] = stride[k] * nbk(k, lv[k]);
+        for (int k = 0; k < D; ++k) {
+          bool empty = false;
+          for (int j = 0; j < D; ++j) {
+            lo[j] = (j == k) ? 0 : nbk(j, lv[j] - 1);
+            hi[j] = nbk(j, lv[j]);
+            if (lo[j] >= hi[j]) empty = true;
+            dig[j] = lo[j];
+          }
+          if (empty) continue;
+          const int nk = hi[k];
+          const int64_t mi = I.md0 + k * L1 + lv[k];
+          const double* Tk = &tm[toff[mi]];
+          const double cut = tcut[mi];
+          const double ck = P->h_ck[I.ck0 + k];
+          int64_t e = 0;
+          for (int j = 0; j < D; ++j) e += dig[j] * stride[j];
+          while (true) {
+            const int32_t r = mp[e];
+            if (r >= 0) {
+              const int ik = dig[k];
+              for (int i2 = 0; i2 < nk; ++i2) {
+                if (i2 == ik || std::fabs(Tk[ik * nk + i2]) <= cut) continue;
+                const int32_t r2 = mp[e + (int64_t)(i2 - ik) * stride[k]];
+                if (r2 < 0) continue;
+                if (sweep == 0) {
+                  ++ptr[r + 1];
+                } else {
+                  col[pos[r]] = r2;
+                  val[pos[r]] = ck * Tk[ik * nk + i2];
+                  ++pos[r];
+                }
+              }
+              if (sweep == 1) diag[r] += ck * Tk[ik * nk + ik];
+            }
+            int j = 0;
+            for (; j < D; ++j) {
+              ++dig[j];
+              e += stride[j];
+              if (dig[j] < hi[j]) break;
+              e -= (int64_t)(dig[j] - lo[j]) * stride[j];
+              dig[j] = lo[j];
+            }
+            if (j == D) break;
+          }
+        }
+      }
+    }
+    if (sweep == 1)
+      for (int64_t r = 0; r < nrows; ++r) {
+        col[ptr[r]] = (int32_t)r;
+        val[ptr[r]] = diag[r];
+        if (pos[r] != ptr[r + 1]) { set_error("internal: kinetic matrix row count"); return SG4_ERR_STATE; }
+      }
+  }
+  int rc;
+  if ((rc = P->d_kptr.alloc(sizeof(int64_t) * (nrows + 1)))) return rc;
+  if ((rc = P->d_kcol.alloc(sizeof(int32_t) * col.size()))) return rc;
+  if ((rc = P->d_kval.alloc(sizeof(double) * val.size()))) return rc;
+  HIP_TRY(hipMemcpy(P->d_kptr.p, ptr.data(), sizeof(int64_t) * (nrows + 1), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(P->d_kcol.p, col.data(), sizeof(int32_t) * col.size(), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(P->d_kval.p, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice));
+  P->kin_nnz = (int64_t)col.size();
+  P->split_ok = true;
+  return SG4_OK;
+}
+
 static int build_terms(sg4_plan* P) {
   P->h_terms.resize(P->nterms);
   std::vector<double> ck = P->h_ck;
@@ -998,7 +1453,7 @@ static int build_terms(sg4_plan* P) {
   if (rc) return rc;
   HIP_TRY(hipMemcpy(P->d_terms.p, P->h_terms.data(), sizeof(TermDev) * P->h_terms.size(),
                     hipMemcpyHostToDevice));
-  return SG4_OK;
+  return build_kinetic(P);
 }
 
 // Work items for a block of nvec vectors: for every term the vectors are cut
@@ -1009,18 +1464,48 @@ static int build_terms(sg4_plan* P) {
 // positions congruent modulo 8 (workgroups are dealt round-robin to the eight
 // XCDs), so the chunks of a term share one L2 for their gathers; terms are
 // given to the XCD with the least accumulated work.
+// Split path (split = true): an item needs (nv + 1) N values of LDS (the
+// chunk and the potential); a term that does not fit the budget with one
+// vector goes to the second list, launched with its own LDS size.
 static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_force = 0,
-                     int budget = LDS_SMALL, int xcd = 0) {
-  const int64_t key = ((((int64_t)nvec * 16 + esize + (nbuf_force ? 1 : 0)) * 1024 +
-                        budget / 1024) << 1) | (xcd ? 1 : 0);
+                     int budget = LDS_SMALL, int xcd = 0, bool split = false) {
+  const int64_t key = (((((int64_t)nvec * 16 + esize + (nbuf_force ? 1 : 0)) * 1024 +
+                         budget / 1024) << 1) | (xcd ? 1 : 0)) * 2 + (split ? 1 : 0);
   auto itp = P->items.find(key);
   if (itp != P->items.end()) { *out = itp->second.get(); return SG4_OK; }
-  struct HI { Item it; double cost; };
+  struct HI { Item it; double cost; bool large; };
   std::vector<HI> all;
   all.reserve(P->nterms * 2);
   int64_t nbig = 0, slot = 0;
-  int lds = 0;
-  for (int64_t t = 0; t < P->nterms; ++t) {
+  int lds = 0, lds2 = 0;
+  if (split) {
+    for (int64_t t = 0; t < P->nterms; ++t) {
+      const TermDev& T = P->h_terms[t];
+      const int64_t n = (T.nact == 0) ? 0 : T.nq;
+      const bool large = 2 * n * esize > budget;
+      const int cap = large ? LDS_MAX : budget;
+      int gmax = 0;
+      while ((1 << (gmax + 1)) <= nvec && ((1 << (gmax + 1)) + 1) * n * esize <= cap) ++gmax;
+      double work = 0;
+      for (int q = 0; q < T.nact; ++q) work += P->h_md[T.mat[q]].nq;
+      work = (work + 1.0) * (double)std::max<int64_t>(n, 1);
+      for (int v0 = 0; v0 < nvec;) {
+        int g = gmax;
+        while ((1 << g) > nvec - v0) --g;
+        HI h{};
+        h.it.term = (int32_t)t;
+        h.it.v0 = (int16_t)v0;
+        h.it.lgnv = (int8_t)g;
+        h.cost = work * (1 << g);
+        h.large = large;
+        int& l = large ? lds2 : lds;
+        l = std::max<int>(l, (int)(((1 << g) + 1) * n * esize));
+        all.push_back(h);
+        v0 += 1 << g;
+      }
+    }
+  }
+  for (int64_t t = 0; t < (split ? 0 : P->nterms); ++t) {
     const TermDev& T = P->h_terms[t];
     const int64_t nmax = std::max(T.nq, T.nb);
     const int nbuf = nbuf_force ? nbuf_force : ((P->type_op == 10) ? P->D + 3 : 2);
@@ -1058,15 +1543,18 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
     }
   }
   std::stable_sort(all.begin(), all.end(), [](const HI& x, const HI& y) { return x.cost > y.cost; });
-  std::vector<Item> h;
-  h.reserve(all.size());
-  if (!xcd) {
-    for (auto& x : all) h.push_back(x.it);
-  } else {
+  auto order = [&](bool large) {
+    std::vector<Item> h;
+    if (!xcd) {
+      for (auto& x : all)
+        if (x.large == large) h.push_back(x.it);
+      return h;
+    }
     std::vector<std::vector<Item>> lane(8);
     double acc[8] = {0};
     std::map<int32_t, int> owner;
     for (auto& x : all) {
+      if (x.large != large) continue;
       auto o = owner.find(x.it.term);
       int l;
       if (o == owner.end()) {
@@ -1084,15 +1572,24 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
     pad.term = -1;
     for (size_t i = 0; i < m; ++i)
       for (int l = 0; l < 8; ++l) h.push_back(i < lane[l].size() ? lane[l][i] : pad);
-  }
+    return h;
+  };
+  const std::vector<Item> h = order(false);
+  const std::vector<Item> h2 = order(true);
   auto ip = std::make_unique<ItemPlan>();
   ip->nitems = (int64_t)h.size();
   ip->nbig = nbig;
   ip->slot_size = slot;
   ip->lds = lds;
+  ip->nitems2 = (int64_t)h2.size();
+  ip->lds2 = lds2;
   int rc = ip->items.alloc(sizeof(Item) * h.size());
   if (rc) return rc;
   HIP_TRY(hipMemcpy(ip->items.p, h.data(), sizeof(Item) * h.size(), hipMemcpyHostToDevice));
+  if (!h2.empty()) {
+    if ((rc = ip->items2.alloc(sizeof(Item) * h2.size()))) return rc;
+    HIP_TRY(hipMemcpy(ip->items2.p, h2.data(), sizeof(Item) * h2.size(), hipMemcpyHostToDevice));
+  }
   if (nbig > 0 && (rc = ip->scratch.alloc((size_t)esize * nbig * slot))) return rc;
   *out = ip.get();
   P->items[key] = std::move(ip);
@@ -1135,6 +1632,10 @@ static int check_apply(sg4_plan* P, int nvec, int prec) {
   return SG4_OK;
 }
 
+static bool use_split(const sg4_plan* P) {
+  return P->type_op == 1 && P->grouping && P->split_ok;
+}
+
 static LaunchCfg cfg_for(sg4_plan* P, int nvec, int prec) {
   auto it = P->tuned.find(nvec * 16 + prec);
   return it == P->tuned.end() ? LaunchCfg{} : it->second;
@@ -1147,7 +1648,8 @@ static int prepare(sg4_plan* P, int nvec, int prec, const LaunchCfg& cfg, ItemPl
   const int esize = (prec == SG4_PREC_FP64) ? 8 : 4;
   if (esize == 4 && (rc = ensure_f32(P))) return rc;
   const bool t10 = P->type_op == 10;
-  return get_items(P, nvec, esize, ip, 0, t10 ? LDS_SMALL : cfg.lds_budget, t10 ? 0 : cfg.xcd);
+  return get_items(P, nvec, esize, ip, 0, t10 ? LDS_SMALL : cfg.lds_budget, t10 ? 0 : cfg.xcd,
+                   use_split(P));
 }
 
 static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, int prec,
@@ -1159,7 +1661,8 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
   if ((rc = prepare(P, nvec, prec, cfg, &ip))) return rc;
   const int esize = (prec == SG4_PREC_FP64) ? 8 : 4;
   const int64_t n = P->nb_ba * nvec;
-  hipLaunchKernelGGL(k_fill, dim3(grid_blocks(n)), dim3(256), 0, P->stream, hpsi, n, 0.0);
+  const bool split = use_split(P);
+  if (!split) hipLaunchKernelGGL(k_fill, dim3(grid_blocks(n)), dim3(256), 0, P->stream, hpsi, n, 0.0);
   KArgs a;
   a.terms = P->d_terms.as<TermDev>();
   a.md = P->d_md.as<MatDev>();
@@ -1171,7 +1674,7 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
   a.items = ip->items.as<Item>();
   a.scratch = ip->scratch.as<double>();
   a.slot_size = ip->slot_size;
-  P->last_items = ip->nitems;
+  P->last_items = ip->nitems + ip->nitems2;
   if (P->timing) {
     if (P->ev_used == P->ev_pairs.size()) {
       hipEvent_t e0, e1;
@@ -1186,7 +1689,28 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
   }
   const unsigned grid = (unsigned)((cfg.grid > 0 && cfg.grid < ip->nitems) ? cfg.grid : ip->nitems);
   const dim3 blk(P->type_op == 10 ? 256 : cfg.nthreads);
-  if (P->type_op == 10) {
+  if (split) {
+    // kinetic rows (initialise hpsi), then the potential part of the terms
+    hipLaunchKernelGGL(k_kin_rows, dim3(grid_blocks(n)), dim3(256), 0, P->stream,
+                       P->d_kptr.as<int64_t>(), P->d_kcol.as<int32_t>(), P->d_kval.as<double>(),
+                       psi, hpsi, P->nb_ba, nvec);
+    for (int list = 0; list < 2; ++list) {
+      const int64_t ni = list ? ip->nitems2 : ip->nitems;
+      if (ni == 0) continue;
+      KArgs b = a;
+      b.nitems = (int)ni;
+      b.items = list ? ip->items2.as<Item>() : ip->items.as<Item>();
+      const unsigned gr = (unsigned)((cfg.grid > 0 && cfg.grid < ni) ? cfg.grid : ni);
+      const int lds = list ? ip->lds2 : ip->lds;
+      if (esize == 8) {
+        KMats<double> km{P->d_mats.as<double>(), P->d_V.as<double>(), P->d_ck.as<double>()};
+        hipLaunchKernelGGL(k_hpsi_t1s_f64, dim3(gr), blk, lds, P->stream, b, km);
+      } else {
+        KMats<float> km{P->d_mats32.as<float>(), P->d_V32.as<float>(), P->d_ck32.as<float>()};
+        hipLaunchKernelGGL(k_hpsi_t1s_f32, dim3(gr), blk, lds, P->stream, b, km);
+      }
+    }
+  } else if (P->type_op == 10) {
     KMats10 km{P->d_mats.as<double>(), P->d_V.as<double>(), P->d_G.as<double>(),
                P->d_Jac.as<double>(), P->d_sqRJ.as<double>()};
     hipLaunchKernelGGL(k_hpsi_t10_f64, dim3((unsigned)ip->nitems), blk, ip->lds, P->stream, a, km);
@@ -1275,6 +1799,10 @@ int sg4_plan_create_batch(int32_t ninst, const sg4_tables* tabs, int32_t max_nve
     (void)hipFuncSetAttribute((const void*)k_hpsi_t1_f64,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
     (void)hipFuncSetAttribute((const void*)k_hpsi_t1_f32,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
+    (void)hipFuncSetAttribute((const void*)k_hpsi_t1s_f64,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
+    (void)hipFuncSetAttribute((const void*)k_hpsi_t1s_f32,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
     (void)hipFuncSetAttribute((const void*)k_hpsi_t10_f64,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
@@ -1381,10 +1909,12 @@ int sg4_plan_create_batch(int32_t ninst, const sg4_tables* tabs, int32_t max_nve
   int rc;
   if ((rc = P->d_mats.alloc(sizeof(double) * off))) return rc;
   HIP_TRY(hipMemcpy(P->d_mats.p, matsR.data(), sizeof(double) * off, hipMemcpyHostToDevice));
+  P->h_mats = matsR;
   if ((rc = P->d_md.alloc(sizeof(MatDev) * P->h_md.size()))) return rc;
   HIP_TRY(hipMemcpy(P->d_md.p, P->h_md.data(), sizeof(MatDev) * P->h_md.size(), hipMemcpyHostToDevice));
   if ((rc = P->d_map.alloc(sizeof(int32_t) * P->sum_nb))) return rc;
   HIP_TRY(hipMemcpy(P->d_map.p, mapv.data(), sizeof(int32_t) * P->sum_nb, hipMemcpyHostToDevice));
+  P->h_map.swap(mapv);
   if ((rc = P->d_levels.alloc(sizeof(int32_t) * P->h_levels.size()))) return rc;
   HIP_TRY(hipMemcpy(P->d_levels.p, P->h_levels.data(), sizeof(int32_t) * P->h_levels.size(), hipMemcpyHostToDevice));
   if ((rc = P->d_qoff.alloc(sizeof(int64_t) * P->h_qoff.size()))) return rc;
@@ -1536,11 +2066,23 @@ int sg4_apply_host(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, i
 
 int sg4_plan_stats(const sg4_plan* p, int32_t nvec, double* flops, double* bytes, int64_t* items) {
   if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
-  if (flops) *flops = (p->type_op == 10 ? 4.0 : 3.0) * p->flops_per_vec * nvec;
-  if (bytes) *bytes = p->bytes_fixed + 16.0 * (double)p->nb_ba * nvec;
+  const bool split = use_split(p);
+  if (flops)
+    *flops = split ? (2.0 * p->flops_per_vec + 2.0 * (double)p->kin_nnz) * nvec
+                   : (p->type_op == 10 ? 4.0 : 3.0) * p->flops_per_vec * nvec;
+  if (bytes)
+    *bytes = p->bytes_fixed + 16.0 * (double)p->nb_ba * nvec +
+             (split ? 12.0 * (double)p->kin_nnz + 8.0 * (double)(p->nb_ba + 1) : 0.0);
   if (items) {
     *items = p->last_items;
   }
+  return SG4_OK;
+}
+
+int sg4_plan_split_info(const sg4_plan* p, int32_t* split, int64_t* kin_nnz) {
+  if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
+  if (split) *split = use_split(p) ? 1 : 0;
+  if (kin_nnz) *kin_nnz = p->kin_nnz;
   return SG4_OK;
 }
 

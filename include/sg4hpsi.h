@@ -182,9 +182,19 @@ int sg4_apply_graph(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, 
 
 /* Plan options.  SG4_OPT_GROUPING (default 1): terms whose active modes are
  * all square go through register-blocked passes over groups of up to three
- * modes; 0 forces one pass per mode (both give the same operator). */
+ * modes, and a type-1 operator whose 1-D quadratures integrate products of
+ * basis functions exactly (B^T W B = I) on a complete Smolyak term set takes
+ * the split path: its kinetic part becomes one sparse matrix over the packed
+ * rows, K[r][r'] = sum_k c_k T_k(L - Lambda_k(r))[i_k][i'_k] with
+ * T_k = BtW D2 B (the combination weights telescope), and the terms keep
+ * only the basis->grid, potential and grid->basis products.  0 forces one
+ * pass per mode with the kinetic on the grid, as in the reference (all paths
+ * give the same operator up to rounding). */
 #define SG4_OPT_GROUPING 1
 int sg4_plan_set_option(sg4_plan *p, int32_t option, int32_t value);
+/* Whether applies of this plan take the split path (split = 1) and the number
+ * of stored entries of its kinetic matrix.  No counterpart in the reference. */
+int sg4_plan_split_info(const sg4_plan *p, int32_t *split, int64_t *kin_nnz);
 
 /* Statistics of the last apply: launches, items, and the algorithmic flop and
  * byte counts per block apply used for roofline reporting. */
