@@ -1029,25 +1029,45 @@ __global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out,
 // ---------------------------------------------------------------------------
 // small kernels
 // ---------------------------------------------------------------------------
-__global__ void k_cols_to_rows(const double* __restrict__ in, double* __restrict__ out,
-                               int64_t nrow, int nvec) {
-  const int64_t n = nrow * nvec;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = i / nvec;
-    const int v = (int)(i - r * nvec);
-    out[i] = in[(int64_t)v * nrow + r];
+// Layout changes between the column-major block (v * nrow + r) and the rows
+// layout (r * nvec + v), through 64-row x 16-vector LDS tiles so that both the
+// reads and the writes are contiguous.  grid = (ceil(nrow/64), ceil(nvec/16)),
+// 256 threads.
+constexpr int TT_R = 64, TT_V = 16;
+__global__ __launch_bounds__(256) void k_cols_to_rows(const double* __restrict__ in,
+                                                      double* __restrict__ out, int64_t nrow,
+                                                      int nvec) {
+  __shared__ double tile[TT_V][TT_R + 1];
+  const int64_t r0 = (int64_t)blockIdx.x * TT_R;
+  const int v0 = blockIdx.y * TT_V;
+  for (int k = threadIdx.x; k < TT_R * TT_V; k += blockDim.x) {
+    const int vv = k / TT_R, rr = k % TT_R;   // read columns: rows contiguous
+    if (r0 + rr < nrow && v0 + vv < nvec) tile[vv][rr] = in[(int64_t)(v0 + vv) * nrow + r0 + rr];
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < TT_R * TT_V; k += blockDim.x) {
+    const int rr = k / TT_V, vv = k % TT_V;   // write rows: vectors contiguous
+    if (r0 + rr < nrow && v0 + vv < nvec) out[(r0 + rr) * nvec + v0 + vv] = tile[vv][rr];
   }
 }
-__global__ void k_rows_to_cols(const double* __restrict__ in, double* __restrict__ out,
-                               int64_t nrow, int nvec) {
-  const int64_t n = nrow * nvec;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int v = (int)(i / nrow);
-    const int64_t r = i - (int64_t)v * nrow;
-    out[i] = in[r * nvec + v];
+__global__ __launch_bounds__(256) void k_rows_to_cols(const double* __restrict__ in,
+                                                      double* __restrict__ out, int64_t nrow,
+                                                      int nvec) {
+  __shared__ double tile[TT_V][TT_R + 1];
+  const int64_t r0 = (int64_t)blockIdx.x * TT_R;
+  const int v0 = blockIdx.y * TT_V;
+  for (int k = threadIdx.x; k < TT_R * TT_V; k += blockDim.x) {
+    const int rr = k / TT_V, vv = k % TT_V;   // read rows: vectors contiguous
+    if (r0 + rr < nrow && v0 + vv < nvec) tile[vv][rr] = in[(r0 + rr) * nvec + v0 + vv];
   }
+  __syncthreads();
+  for (int k = threadIdx.x; k < TT_R * TT_V; k += blockDim.x) {
+    const int vv = k / TT_R, rr = k % TT_R;   // write columns: rows contiguous
+    if (r0 + rr < nrow && v0 + vv < nvec) out[(int64_t)(v0 + vv) * nrow + r0 + rr] = tile[vv][rr];
+  }
+}
+static dim3 tile_grid(int64_t nrow, int nvec) {
+  return dim3((unsigned)((nrow + TT_R - 1) / TT_R), (unsigned)((nvec + TT_V - 1) / TT_V));
 }
 __global__ void k_fill(double* __restrict__ p, int64_t n, double val) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -2039,10 +2059,10 @@ int sg4_apply(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_
   if (nvec == 1) {
     return apply_rows(p, 1, psi, hpsi, prec);
   }
-  hipLaunchKernelGGL(k_cols_to_rows, dim3(grid_blocks(n)), dim3(256), 0, p->stream, psi, ri, p->nb_ba, nvec);
+  hipLaunchKernelGGL(k_cols_to_rows, tile_grid(p->nb_ba, nvec), dim3(256), 0, p->stream, psi, ri, p->nb_ba, nvec);
   int rc = apply_rows(p, nvec, ri, ro, prec);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_rows_to_cols, dim3(grid_blocks(n)), dim3(256), 0, p->stream, ro, hpsi, p->nb_ba, nvec);
+  hipLaunchKernelGGL(k_rows_to_cols, tile_grid(p->nb_ba, nvec), dim3(256), 0, p->stream, ro, hpsi, p->nb_ba, nvec);
   HIP_TRY(hipGetLastError());
   return SG4_OK;
 }
@@ -2152,7 +2172,7 @@ int sg4_plan_autotune(sg4_plan* p, int32_t nvec, int32_t prec, float* best_ms) {
       cus = prop.multiProcessorCount;
   }
   const int threads[] = {256, 512};
-  const int budgets[] = {32 * 1024, 48 * 1024, 64 * 1024, 80 * 1024};
+  const int budgets[] = {32 * 1024, 48 * 1024, 64 * 1024, 80 * 1024, 112 * 1024, 160 * 1024};
   const int grids[] = {0, 4, 8};   // 0: one per item; k: k workgroups per CU, persistent
   for (int nt : threads)
     for (int bu : budgets)

@@ -1,0 +1,227 @@
+"""Wave-packet files in ElVibRot's formats, so that vectors computed on the
+GPU (Davidson eigenvectors, propagated packets) can be read by the Fortran
+code (``read_WP=t``, ``name_file_readWP``) and the other way round.
+
+Formats (SRC/sub_WP/sub_module_psi_io.f90):
+
+* version 0, formatted (the default, ``FilePsiVersion = 0``,
+  SRC/EVR_system_m.f90:104): header line ``nb_psi nb_tot``
+  (Write_header_saveFile_psi :1312-1369), then for every vector one line
+  ``i_1 .. i_D i_h i_e a`` (``a b`` for a complex vector) per basis function
+  with ``|a| >= epsi``, the ``i_k`` being the 1-based mode indices of the
+  packed basis function (Rec_ndim_index, SRC/sub_Basis/sub_module_basis.f90
+  :3004-3084, SG type 4 branch :3055-3065), closed by ``end wp <j>``
+  (Write_Psi_nDBasis :1478-1560).  The reader accepts a one-integer header
+  as well and, like lect_psiBasisRepnotall_nD (:264-), matches the index
+  tuples to packed rows and skips functions the basis does not hold.
+* version 1, formatted: ``&headerFile`` namelist (Version_File, nb_psi,
+  nb_tot), the index list (one ``i_1 .. i_D i_h i_e`` line per packed row,
+  Write_list_nDindBasis :1370-1477), then per vector ``T`` and the
+  ``nb_tot`` coefficients, list-directed (Write_Psi_nDBasis case 1).
+* version 1, unformatted (sequential records with 4-byte length markers):
+  ``'X'``, nb_psi, nb_tot, 1, ``'X'``, nb_tot index records, one record
+  ``T, coefficients`` per vector.
+
+Only one electronic state and no extra index (``i_h = i_e = 1``) occur for
+the SG4 bases here.
+"""
+import struct
+
+import numpy as np
+
+
+def packed_indices(basis):
+    """(nb_ba, D) int32: 0-based mode indices of every packed row, decoded
+    from the term map (every packed row occurs in some term)."""
+    cached = getattr(basis, "_packed_idx", None)
+    if cached is not None:
+        return cached
+    D, lev, mp = basis.D, basis.levels(), basis.map()
+    nbtab = {}
+    out = np.full((basis.nb_ba, D), -1, dtype=np.int32)
+    seen = np.zeros(basis.nb_ba, dtype=bool)
+    off = 0
+    for t in range(basis.nterms):
+        dims = []
+        for k in range(D):
+            key = (k, int(lev[t, k]))
+            if key not in nbtab:
+                nbtab[key] = basis.basis1d(k, key[1])["nb"]
+            dims.append(nbtab[key])
+        n = int(np.prod(dims))
+        rows = mp[off:off + n]
+        off += n
+        new = (rows >= 0) & ~seen[np.maximum(rows, 0)]
+        if not new.any():
+            continue
+        e = np.nonzero(new)[0]
+        r = rows[e]
+        for k in range(D):                 # first index fastest
+            out[r, k] = e % dims[k]
+            e = e // dims[k]
+        seen[r] = True
+    if not seen.all():
+        raise ValueError("some packed rows occur in no term")
+    basis._packed_idx = out
+    return out
+
+
+def _fmt_real(x):
+    return f"{x: .17E}"
+
+
+def write_psi(path, psi, basis, version=0, epsi=0.0, T=0.0, formatted=True):
+    """Write the vectors ``psi`` (nb_ba x n, real or complex) as a wave-packet
+    file (sub_save_psi, SRC/sub_WP/sub_module_psi_io.f90:65-124)."""
+    psi = np.asarray(psi)
+    if psi.ndim == 1:
+        psi = psi[:, None]
+    nb, n = psi.shape
+    if nb != basis.nb_ba:
+        raise ValueError("psi rows do not match the basis")
+    idx = packed_indices(basis) + 1
+    cplx = np.iscomplexobj(psi)
+    if not formatted:
+        if version != 1:
+            raise ValueError("unformatted files are written with version 1 only")
+        with open(path, "wb") as f:
+            def rec(b):
+                f.write(struct.pack("<i", len(b)) + b + struct.pack("<i", len(b)))
+            rec(b"X")
+            rec(struct.pack("<i", n))
+            rec(struct.pack("<i", nb))
+            rec(struct.pack("<i", 1))
+            rec(b"X")
+            ones = np.ones((nb, 2), dtype=np.int32)
+            full = np.ascontiguousarray(np.hstack([idx, ones]).astype("<i4"))
+            for r in range(nb):
+                rec(full[r].tobytes())
+            for j in range(n):
+                v = psi[:, j].astype("<c16" if cplx else "<f8")
+                rec(struct.pack("<d", T) + v.tobytes())
+        return
+    with open(path, "w") as f:
+        if version == 0:
+            f.write(f" {n} {nb}\n")
+            for j in range(n):
+                v = psi[:, j]
+                keep = np.abs(v) >= epsi
+                for r in np.nonzero(keep)[0]:
+                    ind = " ".join(str(int(i)) for i in idx[r])
+                    if cplx:
+                        f.write(f" {ind} 1 1 {_fmt_real(v[r].real)} {_fmt_real(v[r].imag)}\n")
+                    else:
+                        f.write(f" {ind} 1 1 {_fmt_real(v[r])}\n")
+                f.write(f" end wp {j + 1}\n")
+        elif version == 1:
+            f.write(f"&HEADERFILE\n VERSION_FILE=1,\n NB_PSI={n},\n NB_TOT={nb},\n /\n")
+            for r in range(nb):
+                f.write(" " + " ".join(str(int(i)) for i in idx[r]) + " 1 1\n")
+            for j in range(n):
+                v = psi[:, j]
+                vals = [T] + ([x for c in v for x in (c.real, c.imag)] if cplx else list(v))
+                for a in range(0, len(vals), 4):
+                    f.write(" " + " ".join(_fmt_real(x) for x in vals[a:a + 4]) + "\n")
+        else:
+            raise ValueError("version must be 0 or 1")
+
+
+def _rows_of(basis):
+    idx = packed_indices(basis) + 1
+    return {tuple(int(i) for i in r): n for n, r in enumerate(idx)}
+
+
+def read_psi(path, basis, cplx=False, formatted=True):
+    """Read a wave-packet file into an (nb_ba, n) array (complex when
+    ``cplx``).  Returns (psi, info) with info = dict(version, nb_psi, nb_tot,
+    T (list, version 1), skipped (functions absent from the basis))."""
+    D = basis.D
+    nbr = basis.nb_ba
+    dt = np.complex128 if cplx else np.float64
+    info = dict(skipped=0, T=[])
+    if not formatted:
+        with open(path, "rb") as f:
+            data = f.read()
+        recs, p = [], 0
+        while p < len(data):
+            (ln,) = struct.unpack_from("<i", data, p)
+            recs.append(data[p + 4:p + 4 + ln])
+            p += 8 + ln
+        if recs[0] != b"X":
+            raise ValueError("not a version-1 unformatted wave-packet file")
+        n = struct.unpack("<i", recs[1])[0]
+        nb = struct.unpack("<i", recs[2])[0]
+        info.update(version=struct.unpack("<i", recs[3])[0], nb_psi=n, nb_tot=nb)
+        lst = [np.frombuffer(recs[5 + r], dtype="<i4") for r in range(nb)]
+        rows = _rows_of(basis)
+        target = np.array([rows.get(tuple(int(i) for i in x[:D]), -1) for x in lst])
+        psi = np.zeros((nbr, n), dtype=dt)
+        for j in range(n):
+            b = recs[5 + nb + j]
+            info["T"].append(struct.unpack_from("<d", b, 0)[0])
+            v = np.frombuffer(b, dtype="<c16" if cplx else "<f8", offset=8)
+            ok = target >= 0
+            psi[target[ok], j] = v[ok]
+            info["skipped"] += int((~ok).sum())
+        return psi, info
+    with open(path) as f:
+        lines = f.read().splitlines()
+    first = lines[0].split()
+    rows = _rows_of(basis)
+    try:
+        ints = [int(x) for x in first]
+    except ValueError:
+        ints = None
+    if ints:
+        n = ints[0]
+        nb = ints[1] if len(ints) > 1 else 0
+        info.update(version=0, nb_psi=n, nb_tot=nb)
+        psi = np.zeros((nbr, n), dtype=dt)
+        j, pos = 0, 1
+        while j < n and pos < len(lines):
+            line = lines[pos]
+            pos += 1
+            if "end" in line:
+                j += 1
+                continue
+            tok = line.split()
+            if not tok:
+                continue
+            ind = tuple(int(x) for x in tok[:D])
+            vals = [float(x.replace("D", "E").replace("d", "e")) for x in tok[D + 2:]]
+            r = rows.get(ind)
+            if r is None:
+                info["skipped"] += 1
+                continue
+            psi[r, j] = complex(vals[0], vals[1]) if cplx else vals[0]
+        return psi, info
+    # version 1: &headerFile namelist, possibly over several lines
+    if not lines[0].lstrip().startswith("&"):
+        raise ValueError("unrecognised wave-packet file header")
+    text, pos = [], 0
+    while True:
+        text.append(lines[pos])
+        pos += 1
+        if "/" in text[-1]:
+            break
+    hdr = " ".join(text).upper().replace(",", " ")
+    kv = dict(tok.split("=") for tok in hdr.split() if "=" in tok)
+    n, nb = int(kv["NB_PSI"]), int(kv["NB_TOT"])
+    info.update(version=int(kv.get("VERSION_FILE", 1)), nb_psi=n, nb_tot=nb)
+    lst = [tuple(int(x) for x in lines[pos + r].split()[:D]) for r in range(nb)]
+    pos += nb
+    target = np.array([rows.get(t, -1) for t in lst])
+    toks = " ".join(lines[pos:]).replace("D", "E").split()
+    vals = np.array([float(x) for x in toks])
+    per = 1 + (2 * nb if cplx else nb)
+    psi = np.zeros((nbr, n), dtype=dt)
+    ok = target >= 0
+    for j in range(n):
+        blk = vals[j * per:(j + 1) * per]
+        info["T"].append(float(blk[0]))
+        v = blk[1:]
+        if cplx:
+            v = v[0::2] + 1j * v[1::2]
+        psi[target[ok], j] = v[ok]
+        info["skipped"] += int((~ok).sum())
+    return psi, info
