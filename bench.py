@@ -6,14 +6,18 @@ HO bases nq(l) = nb(l) = 1 + 2l, block of 16 seeded orthonormal vectors.
 A step is one block application H.psi (sub_TabOpPsi_FOR_SGtype4) with the
 input block already resident in HBM.
 
-Prints ONE JSON line with the metric, the roofline of the H.psi kernel
-(algorithmic flops per launch over the launch time measured with HIP events on
-the plan's stream) and the CPU baseline (the oracle, OpenMP over terms, on a
-bounded sample of the same workload).
+Prints ONE JSON line with the metric, the roofline of the dominant kernel
+(the term kernel's main launch: algorithmic flops per launch over the launch
+time measured with HIP events on the plan's stream; HBM traffic per launch
+from the committed rocprofv3 counter summary when one exists for this
+workload) and the CPU baseline (the oracle, OpenMP over terms, on a bounded
+sample of the same workload).  Launch parameters are autotuned before the
+timed region (--no-autotune keeps the defaults).
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -27,9 +31,10 @@ PREC = {"fp64": 0, "f32": 1, "f32x2": 2}
 DTYPE = {"fp64": "f64", "f32": "f32", "f32x2": "f32x2"}
 
 
-def cpu_baseline(D, L, nvec, stride):
-    """Oracle (OpenMP) on every `stride`-th term of the same workload; the
-    rate is scaled by the sampled share of the algorithmic flops."""
+def cpu_baseline(D, L, nvec, stride, min_seconds=10.0):
+    """Oracle (OpenMP over terms) on every `stride`-th term of the same
+    workload, repeated until at least `min_seconds` of CPU work (at most 5
+    times); the rate is scaled by the sampled share of the algorithmic flops."""
     from oracle import oracle as O
     o = O.SG4(D, L)
     V = o.grid_hh(0.111803)
@@ -37,13 +42,27 @@ def cpu_baseline(D, L, nvec, stride):
     per_term = np.array([2.0 * nq[t] * np.sum(1 + 2 * o.lev[t]) for t in range(o.nterms)])
     frac = per_term[::stride].sum() / per_term.sum()
     psi = np.random.default_rng(2026).standard_normal((o.nb_ba, nvec))
-    t0 = time.perf_counter()
-    o.hpsi(psi, V, -0.5 * np.ones(D), t0=0, t1=-1, tstride=stride)
-    dt = time.perf_counter() - t0
-    return {"value": nvec * frac / dt, "unit": "H.psi/s", "cores": O.num_threads(),
+    reps, dt = 0, 0.0
+    while reps < 5 and (reps == 0 or dt < min_seconds):
+        t0 = time.perf_counter()
+        o.hpsi(psi, V, -0.5 * np.ones(D), t0=0, t1=-1, tstride=stride)
+        dt += time.perf_counter() - t0
+        reps += 1
+    return {"value": reps * nvec * frac / dt, "unit": "H.psi/s", "cores": O.num_threads(),
             "kind": "port",
-            "sample": f"every {stride}th Smolyak term ({frac:.3f} of the flops) of the same "
-                      f"{D}D L={L} block-{nvec} FP64 apply, {dt:.1f} s, scaled by the flop share"}
+            "sample": f"{reps} x every {stride}th Smolyak term ({frac:.3f} of the flops) of the "
+                      f"same {D}D L={L} block-{nvec} FP64 apply, {dt:.1f} s in total, "
+                      f"scaled by the flop share"}
+
+
+def measured_traffic(kernel, D, L, nvec):
+    """HBM bytes per launch of `kernel` from profiles/traffic_<kernel>_<D>D_L<L>_block<nvec>.md
+    (line 'traffic_bytes_per_launch: <n>'), or None."""
+    f = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{D}D_L{L}_block{nvec}.md")
+    if not os.path.exists(f):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(f).read())
+    return float(m.group(1)) if m else None
 
 
 def main():
@@ -53,12 +72,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--prec", default="fp64", choices=list(PREC),
                     help="precision of the reported value; the other mode is timed beside it")
-    ap.add_argument("--autotune", action="store_true", help="autotune launch parameters first")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
+    ap.add_argument("--launch", default=None,
+                    help="threads,lds_budget,grid,xcd for the reported precision (skips its autotune)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (x5), for scaling HBM counters")
     ap.add_argument("--no-other", action="store_true", help="do not time the other precision")
     ap.add_argument("--D", type=int, default=10)
     ap.add_argument("--L", type=int, default=8)
     ap.add_argument("--nvec", type=int, default=16)
-    ap.add_argument("--cpu-stride", type=int, default=2)
+    ap.add_argument("--cpu-stride", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -70,6 +93,16 @@ def main():
     import torch
     import elvibrot_amd as E
 
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 8
+        a = torch.ones(n, device="cuda:0", dtype=torch.float64)
+        c = torch.empty_like(a)
+        for _ in range(5):
+            c.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes_read": n * 8, "bytes_written": n * 8, "launches": 5}))
+        return
+
     D, L, nvec = args.D, args.L, args.nvec
     b = E.SG4Basis(D, L)
     op = E.SG4Operator(b, max_nvec=nvec)
@@ -79,8 +112,12 @@ def main():
     Xc = X.T.contiguous()                            # (nvec, nb_ba) = column-major block
     Y = torch.empty_like(Xc)
     prec = PREC[args.prec]
-    if args.autotune:
+    if args.launch:
+        t_, b_, g_, x_ = (int(v) for v in args.launch.split(","))
+        op.set_config(nvec, prec, t_, b_, g_, x_)
+    elif not args.no_autotune:
         op.autotune(nvec, prec)
+    split = op.split_info()[0]
 
     def timed(p, steps, warmup):
         for _ in range(warmup):
@@ -103,11 +140,19 @@ def main():
     other = None
     if not args.no_other:
         op_name = "fp64" if args.prec != "fp64" else "f32"
-        if args.autotune:
+        if not args.no_autotune:
             op.autotune(nvec, PREC[op_name])
         dto, kavgo = timed(PREC[op_name], max(3, args.steps // 2), 2)
         other = {"precision": op_name, "H·psi/s": nvec * max(3, args.steps // 2) / dto,
                  "kernel_ms": kavgo * 1e3}
+    kname = ("k_hpsi_t1s_" if split else "k_hpsi_t1_") + ("f64" if args.prec == "fp64" else "f32")
+    traffic = measured_traffic(kname, D, L, nvec)
+    # flops of the reference's algorithm (B->G, kinetic on the grid, G->B):
+    # F1 = 3 sum_t S_t per vector, S_t = sum_k 2 N_t n_k
+    lev = b.levels()
+    ntab = {(k, l): b.basis1d(k, l)["nq"] for k in range(D) for l in range(L + 1)}
+    nq = np.array([[ntab[k, int(l)] for k, l in enumerate(row)] for row in lev], dtype=np.float64)
+    F1 = 3.0 * float(np.sum(2.0 * np.prod(nq, axis=1) * nq.sum(axis=1))) * nvec
     line = {
         "metric": "H·psi/s 10D Smolyak L=8 block-16: split-FP32 MFMA vs FP64, @1e-8 rel eigen err",
         "value": nvec * args.steps / dt,
@@ -126,11 +171,13 @@ def main():
                    "precision": args.prec, "items": st["items"],
                    "launch": op.config(nvec, prec)},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK[args.prec],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK[args.prec], "traffic": None,
-                     "kernel": "k_hpsi_t1_f64" if args.prec == "fp64" else "k_hpsi_t1_f32",
+                     "unit": "TFLOP/s", "frac": achieved / PEAK[args.prec], "traffic": traffic,
+                     "kernel": kname, "split_path": split,
                      "kernel_ms": kavg * 1e3,
                      "flops_per_launch": st["flops"], "algorithmic_bytes": st["bytes"],
                      "hbm_GBps_algorithmic": st["bytes"] / kavg / 1e9},
+        "reference_algorithm": {"flops_per_apply": F1,
+                                "effective_TFLOPs": F1 / (dt / args.steps) / 1e12},
     }
     if other is not None:
         line["other_precision"] = other

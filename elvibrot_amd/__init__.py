@@ -208,6 +208,11 @@ class SG4Operator:
         _capi.check(lib().sg4_plan_config(self._h, nvec, prec, c))
         return dict(nthreads=c[0], lds_budget=c[1], grid=c[2], xcd=c[3])
 
+    def set_config(self, nvec, prec=PREC_FP64, nthreads=256, lds_budget=65536, grid=0, xcd=0):
+        """Launch parameters for (nvec, prec), e.g. from an earlier autotune."""
+        c = (C.c_int32 * 4)(nthreads, lds_budget, grid, xcd)
+        _capi.check(lib().sg4_plan_set_config(self._h, nvec, prec, c))
+
     def apply_graph(self, psi, out, prec=PREC_FP64):
         """apply_rows through a captured HIP graph (rows layout tensors)."""
         nvec = psi.shape[1]

@@ -644,26 +644,23 @@ __device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km
   }
 }
 
-__global__ __launch_bounds__(NT) void k_hpsi_t1s_f64(KArgs a, KMats<double> km) {
-  extern __shared__ double lds[];
+template <class T>
+__device__ __forceinline__ void split_items(const KArgs& a, const KMats<T>& km) {
+  extern __shared__ double lds_d[];
+  T* lds = (T*)lds_d;
   for (int w = blockIdx.x; w < a.nitems; w += gridDim.x) {
     if (w != (int)blockIdx.x) __syncthreads();   // LDS reuse between items
     const Item it = a.items[w];
     if (it.term < 0) continue;
-    term_t1_split<double>(a, km, it, lds);
+    term_t1_split<T>(a, km, it, lds);
   }
 }
-
-__global__ __launch_bounds__(NT) void k_hpsi_t1s_f32(KArgs a, KMats<float> km) {
-  extern __shared__ double lds_d[];
-  float* lds = (float*)lds_d;
-  for (int w = blockIdx.x; w < a.nitems; w += gridDim.x) {
-    if (w != (int)blockIdx.x) __syncthreads();
-    const Item it = a.items[w];
-    if (it.term < 0) continue;
-    term_t1_split<float>(a, km, it, lds);
-  }
-}
+// the main item list, and (separate names for profiling) the terms that need
+// more LDS than the budget
+__global__ __launch_bounds__(NT) void k_hpsi_t1s_f64(KArgs a, KMats<double> km) { split_items(a, km); }
+__global__ __launch_bounds__(NT) void k_hpsi_t1s_f32(KArgs a, KMats<float> km) { split_items(a, km); }
+__global__ __launch_bounds__(NT) void k_hpsi_t1s_big_f64(KArgs a, KMats<double> km) { split_items(a, km); }
+__global__ __launch_bounds__(NT) void k_hpsi_t1s_big_f32(KArgs a, KMats<float> km) { split_items(a, km); }
 
 // Kinetic part of the split path: hpsi(r, v) = sum_e K(r, col_e) psi(col_e, v)
 // (CSR rows, the diagonal first); one thread per (row, vector), plain stores,
@@ -1141,6 +1138,7 @@ struct ItemPlan {
   DevBuf items2;
   int64_t nitems2 = 0;
   int lds2 = 0;
+  double flops_main = 0;   // algorithmic flops of the main H.psi launch
 };
 
 // Launch configuration of the H.psi kernel; chosen by sg4_plan_autotune.
@@ -1188,6 +1186,7 @@ struct sg4_plan {
   std::vector<GraphEntry> graphs;
   double flops_per_vec = 0, bytes_fixed = 0;
   int64_t last_items = -1;   // work items of the last apply
+  double last_flops = 0;     // algorithmic flops of the last apply's main H.psi launch
   bool grouping = true;      // register-blocked group passes for square terms
   // split type-1 path: kinetic sparse matrix K (CSR over packed rows) and the
   // host copies of the map and matrices it is assembled from
@@ -1495,6 +1494,7 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
   if (itp != P->items.end()) { *out = itp->second.get(); return SG4_OK; }
   struct HI { Item it; double cost; bool large; };
   std::vector<HI> all;
+  double flops_main = 0;
   all.reserve(P->nterms * 2);
   int64_t nbig = 0, slot = 0;
   int lds = 0, lds2 = 0;
@@ -1518,6 +1518,11 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
         h.it.lgnv = (int8_t)g;
         h.cost = work * (1 << g);
         h.large = large;
+        if (!large) {   // basis->grid + grid->basis products and the potential
+          double f = 0;
+          for (int q = 0; q < T.nact; ++q) f += 4.0 * (double)n * P->h_md[T.mat[q]].nq;
+          flops_main += (f + (double)std::max<int64_t>(n, 1)) * (1 << g);
+        }
         int& l = large ? lds2 : lds;
         l = std::max<int>(l, (int)(((1 << g) + 1) * n * esize));
         all.push_back(h);
@@ -1603,6 +1608,8 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
   ip->lds = lds;
   ip->nitems2 = (int64_t)h2.size();
   ip->lds2 = lds2;
+  ip->flops_main = split ? flops_main
+                         : (P->type_op == 10 ? 4.0 : 3.0) * P->flops_per_vec * nvec;
   int rc = ip->items.alloc(sizeof(Item) * h.size());
   if (rc) return rc;
   HIP_TRY(hipMemcpy(ip->items.p, h.data(), sizeof(Item) * h.size(), hipMemcpyHostToDevice));
@@ -1695,6 +1702,7 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
   a.scratch = ip->scratch.as<double>();
   a.slot_size = ip->slot_size;
   P->last_items = ip->nitems + ip->nitems2;
+  P->last_flops = ip->flops_main;
   if (P->timing) {
     if (P->ev_used == P->ev_pairs.size()) {
       hipEvent_t e0, e1;
@@ -1705,7 +1713,7 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
     P->ev0 = P->ev_pairs[P->ev_used].first;
     P->ev1 = P->ev_pairs[P->ev_used].second;
     ++P->ev_used;
-    HIP_TRY(hipEventRecord(P->ev0, P->stream));
+    if (!split) HIP_TRY(hipEventRecord(P->ev0, P->stream));
   }
   const unsigned grid = (unsigned)((cfg.grid > 0 && cfg.grid < ip->nitems) ? cfg.grid : ip->nitems);
   const dim3 blk(P->type_op == 10 ? 256 : cfg.nthreads);
@@ -1716,19 +1724,29 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
                        psi, hpsi, P->nb_ba, nvec);
     for (int list = 0; list < 2; ++list) {
       const int64_t ni = list ? ip->nitems2 : ip->nitems;
-      if (ni == 0) continue;
+      if (ni == 0) {
+        if (P->timing && list == 0) {
+          HIP_TRY(hipEventRecord(P->ev0, P->stream));
+          HIP_TRY(hipEventRecord(P->ev1, P->stream));
+        }
+        continue;
+      }
       KArgs b = a;
       b.nitems = (int)ni;
       b.items = list ? ip->items2.as<Item>() : ip->items.as<Item>();
       const unsigned gr = (unsigned)((cfg.grid > 0 && cfg.grid < ni) ? cfg.grid : ni);
       const int lds = list ? ip->lds2 : ip->lds;
+      if (P->timing && list == 0) HIP_TRY(hipEventRecord(P->ev0, P->stream));
       if (esize == 8) {
         KMats<double> km{P->d_mats.as<double>(), P->d_V.as<double>(), P->d_ck.as<double>()};
-        hipLaunchKernelGGL(k_hpsi_t1s_f64, dim3(gr), blk, lds, P->stream, b, km);
+        hipLaunchKernelGGL(list ? k_hpsi_t1s_big_f64 : k_hpsi_t1s_f64, dim3(gr), blk, lds,
+                           P->stream, b, km);
       } else {
         KMats<float> km{P->d_mats32.as<float>(), P->d_V32.as<float>(), P->d_ck32.as<float>()};
-        hipLaunchKernelGGL(k_hpsi_t1s_f32, dim3(gr), blk, lds, P->stream, b, km);
+        hipLaunchKernelGGL(list ? k_hpsi_t1s_big_f32 : k_hpsi_t1s_f32, dim3(gr), blk, lds,
+                           P->stream, b, km);
       }
+      if (P->timing && list == 0) HIP_TRY(hipEventRecord(P->ev1, P->stream));
     }
   } else if (P->type_op == 10) {
     KMats10 km{P->d_mats.as<double>(), P->d_V.as<double>(), P->d_G.as<double>(),
@@ -1741,7 +1759,7 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
     KMats<float> km{P->d_mats32.as<float>(), P->d_V32.as<float>(), P->d_ck32.as<float>()};
     hipLaunchKernelGGL(k_hpsi_t1_f32, dim3(grid), blk, ip->lds, P->stream, a, km);
   }
-  if (P->timing) HIP_TRY(hipEventRecord(P->ev1, P->stream));
+  if (P->timing && !split) HIP_TRY(hipEventRecord(P->ev1, P->stream));
   HIP_TRY(hipGetLastError());
   return SG4_OK;
 }
@@ -1823,6 +1841,10 @@ int sg4_plan_create_batch(int32_t ninst, const sg4_tables* tabs, int32_t max_nve
     (void)hipFuncSetAttribute((const void*)k_hpsi_t1s_f64,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
     (void)hipFuncSetAttribute((const void*)k_hpsi_t1s_f32,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
+    (void)hipFuncSetAttribute((const void*)k_hpsi_t1s_big_f64,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
+    (void)hipFuncSetAttribute((const void*)k_hpsi_t1s_big_f32,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
     (void)hipFuncSetAttribute((const void*)k_hpsi_t10_f64,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
@@ -2086,13 +2108,10 @@ int sg4_apply_host(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, i
 
 int sg4_plan_stats(const sg4_plan* p, int32_t nvec, double* flops, double* bytes, int64_t* items) {
   if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
-  const bool split = use_split(p);
   if (flops)
-    *flops = split ? (2.0 * p->flops_per_vec + 2.0 * (double)p->kin_nnz) * nvec
-                   : (p->type_op == 10 ? 4.0 : 3.0) * p->flops_per_vec * nvec;
-  if (bytes)
-    *bytes = p->bytes_fixed + 16.0 * (double)p->nb_ba * nvec +
-             (split ? 12.0 * (double)p->kin_nnz + 8.0 * (double)(p->nb_ba + 1) : 0.0);
+    *flops = p->last_items >= 0 ? p->last_flops
+                                : (p->type_op == 10 ? 4.0 : 3.0) * p->flops_per_vec * nvec;
+  if (bytes) *bytes = p->bytes_fixed + 16.0 * (double)p->nb_ba * nvec;
   if (items) {
     *items = p->last_items;
   }
@@ -2210,6 +2229,20 @@ int sg4_plan_config(const sg4_plan* p, int32_t nvec, int32_t prec, int32_t* cfg4
   auto it = p->tuned.find(nvec * 16 + prec);
   const LaunchCfg c = it == p->tuned.end() ? LaunchCfg{} : it->second;
   cfg4[0] = c.nthreads; cfg4[1] = c.lds_budget; cfg4[2] = c.grid; cfg4[3] = c.xcd;
+  return SG4_OK;
+}
+
+int sg4_plan_set_config(sg4_plan* p, int32_t nvec, int32_t prec, const int32_t* cfg4) {
+  if (!p || !cfg4) { set_error("null argument"); return SG4_ERR_ARG; }
+  if ((cfg4[0] != 256 && cfg4[0] != 512) || cfg4[1] < 1024 || cfg4[1] > LDS_MAX || cfg4[2] < 0 ||
+      (cfg4[3] != 0 && cfg4[3] != 1)) {
+    set_error("launch configuration out of range (threads 256/512, 1 KiB <= LDS budget <= "
+              "160 KiB, grid >= 0, xcd 0/1)");
+    return SG4_ERR_ARG;
+  }
+  LaunchCfg c;
+  c.nthreads = cfg4[0]; c.lds_budget = cfg4[1]; c.grid = cfg4[2]; c.xcd = cfg4[3];
+  p->tuned[nvec * 16 + prec] = c;
   return SG4_OK;
 }
 

@@ -175,6 +175,11 @@ int sg4_grad_V(sg4_plan *p, int32_t nvec, const double *phi, const double *psi, 
 int sg4_plan_autotune(sg4_plan *p, int32_t nvec, int32_t prec, float *best_ms);
 /* the launch parameters in use for (nvec, prec): threads, LDS budget, grid, xcd */
 int sg4_plan_config(const sg4_plan *p, int32_t nvec, int32_t prec, int32_t *cfg4);
+/* set the launch parameters for (nvec, prec) (e.g. a configuration saved from
+ * an earlier autotune): threads 256 or 512, LDS budget in bytes (<= 160 KiB),
+ * grid (0 = one workgroup per item, else the persistent grid size), xcd 0/1.
+ * No counterpart in the reference. */
+int sg4_plan_set_config(sg4_plan *p, int32_t nvec, int32_t prec, const int32_t *cfg4);
 /* sg4_apply_rows through a captured HIP graph (zeroing + H.psi kernel), one
  * graph per (nvec, prec, psi, hpsi); replays are ordered after the plan's
  * stream and the plan's stream waits for them. */
@@ -196,12 +201,16 @@ int sg4_plan_set_option(sg4_plan *p, int32_t option, int32_t value);
  * of stored entries of its kinetic matrix.  No counterpart in the reference. */
 int sg4_plan_split_info(const sg4_plan *p, int32_t *split, int64_t *kin_nnz);
 
-/* Statistics of the last apply: launches, items, and the algorithmic flop and
- * byte counts per block apply used for roofline reporting. */
+/* Statistics of the last apply for roofline reporting: its work items, and
+ * the algorithmic flops and HBM bytes of its main H.psi launch (the one that
+ * sg4_plan_timing brackets): basis->grid, grid->basis (and, off the split
+ * path, kinetic) mode products of the active modes; bytes = potential grid,
+ * map, one read of psi and one write of H.psi. */
 int sg4_plan_stats(const sg4_plan *p, int32_t nvec, double *flops, double *bytes,
                    int64_t *items);
 /* Kernel timing for roofline reporting: when enabled, every apply records a
- * pair of HIP events on the plan's stream around its H.psi kernel launch;
+ * pair of HIP events on the plan's stream around its main H.psi kernel launch
+ * (on the split path the term kernel's main item list, not the kinetic rows);
  * sg4_plan_kernel_times waits for them, returns the elapsed milliseconds of
  * up to max recorded launches in order, and clears the record. */
 int sg4_plan_timing(sg4_plan *p, int32_t enable);
