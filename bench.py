@@ -75,6 +75,12 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
     ap.add_argument("--launch", default=None,
                     help="threads,lds_budget,grid,xcd for the reported precision (skips its autotune)")
+    ap.add_argument("--davidson", type=int, default=0, metavar="NLEV",
+                    help="only run the block Davidson for NLEV levels in FP64 and in FP32->FP64 "
+                         "phases at the same workload and report time, H.psi count and the "
+                         "largest relative eigenvalue difference")
+    ap.add_argument("--host-entry", action="store_true",
+                    help="also time sg4_apply_host (host buffers, PCIe included)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (x5), for scaling HBM counters")
     ap.add_argument("--no-other", action="store_true", help="do not time the other precision")
@@ -106,6 +112,27 @@ def main():
     D, L, nvec = args.D, args.L, args.nvec
     b = E.SG4Basis(D, L)
     op = E.SG4Operator(b, max_nvec=nvec)
+    if args.davidson:
+        from elvibrot_amd.davidson import sub_propagation_Davidson
+        out = {"workload": f"{D}D Henon-Heiles SG4 L={L} Davidson, {args.davidson} levels, block {nvec}"}
+        for name, p in (("fp64", PREC["fp64"]), ("f32_then_fp64", PREC["f32"])):
+            if not args.no_autotune:
+                op.autotune(nvec, p)
+                op.autotune(nvec, PREC["fp64"])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = sub_propagation_Davidson(op, args.davidson, max_it=400, conv_resi=1e-6,
+                                         conv_ene=1e-10, prec=p)
+            torch.cuda.synchronize()
+            out[name] = {"seconds": time.perf_counter() - t0, "iterations": r.it,
+                         "n_hpsi": r.n_hpsi, "n_hpsi_f32": getattr(r, "n_hpsi_low", 0),
+                         "converged": bool(np.all(r.converged)), "E": [float(e) for e in r.E]}
+        e64 = np.array(out["fp64"]["E"])
+        e32 = np.array(out["f32_then_fp64"]["E"])
+        out["max_rel_eigen_diff"] = float(np.max(np.abs(e32 - e64) / np.abs(e64)))
+        out["speedup_time_to_solution"] = out["fp64"]["seconds"] / out["f32_then_fp64"]["seconds"]
+        print(json.dumps(out), flush=True)
+        return
     g = torch.Generator(device="cuda:0").manual_seed(2026)
     X = torch.randn(b.nb_ba, nvec, device="cuda:0", dtype=torch.float64, generator=g)
     X, _ = torch.linalg.qr(X)                       # orthonormal block
@@ -182,6 +209,14 @@ def main():
     if other is not None:
         line["other_precision"] = other
         line["speedup_vs_other"] = (nvec * args.steps / dt) / other["H·psi/s"]
+    if args.host_entry:
+        hp = np.ascontiguousarray(X.cpu().numpy())
+        op.apply_host(hp, prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            op.apply_host(hp, prec)
+        line["host_entry_H·psi/s"] = nvec * 5 / (time.perf_counter() - t0)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(D, L, nvec, args.cpu_stride)
     print(json.dumps(line), flush=True)

@@ -76,6 +76,12 @@ class SG4Basis:
     def map(self):
         return np.ctypeslib.as_array(self.tables.map, (self.sum_nb,)).copy()
 
+    def packed(self):
+        """(nb_ba, D) int32: 0-based mode indices of every packed row."""
+        out = np.zeros((self.nb_ba, self.D), dtype=np.int32)
+        _capi.check(lib().sg4_setup_packed(self._h, out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
     def term_grid(self, t):
         lv = np.ctypeslib.as_array(self.tables.levels, (self.nterms, self.D))[t]
         nq = 1

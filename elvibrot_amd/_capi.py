@@ -15,7 +15,7 @@ BASIS_HO, BASIS_LEGENDRE = 0, 1
 # every symbol the header declares (checked by the CPU tests)
 EXPORTS = [
     "sg4_last_error", "sg4_setup_create", "sg4_setup_sizes", "sg4_setup_tables",
-    "sg4_setup_term_grid", "sg4_setup_basis1d", "sg4_setup_destroy", "sg4_plan_create",
+    "sg4_setup_term_grid", "sg4_setup_basis1d", "sg4_setup_destroy", "sg4_setup_packed", "sg4_plan_create",
     "sg4_plan_create_batch",
     "sg4_plan_set_stream", "sg4_plan_set_operator", "sg4_plan_set_henon_heiles", "sg4_apply",
     "sg4_apply_host", "sg4_apply_rows", "sg4_plan_stats", "sg4_plan_timing",
@@ -82,6 +82,7 @@ def lib():
         L.sg4_plan_set_config.argtypes = [vp, i32, i32, C.POINTER(i32)]
         L.sg4_apply_graph.argtypes = [vp, i32, vp, vp, i32]
         L.sg4_plan_set_option.argtypes = [vp, i32, i32]
+        L.sg4_setup_packed.argtypes = [vp, C.POINTER(i32)]
         L.sg4_plan_split_info.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_int64)]
         L.sg4_plan_destroy.argtypes = [vp]
         _lib = L

@@ -1823,6 +1823,36 @@ int sg4_setup_basis1d(const sg4_setup* s, int32_t k, int32_t l, int32_t* nq, int
   return SG4_OK;
 }
 
+int sg4_setup_packed(const sg4_setup* s, int32_t* idx) {
+  if (!s || !idx) { set_error("null argument"); return SG4_ERR_ARG; }
+  const Setup& S = *s->s;
+  const int D = S.D, L1 = S.L + 1;
+  std::vector<char> seen((size_t)S.nb_ba, 0);
+  int64_t nseen = 0;
+  int64_t dims[SG4_MAX_D];
+  for (int64_t t = 0; t < S.nterms && nseen < S.nb_ba; ++t) {
+    int64_t n = 1;
+    for (int k = 0; k < D; ++k) {
+      dims[k] = S.nb_tab[(size_t)k * L1 + S.levels[(size_t)t * D + k]];
+      n *= dims[k];
+    }
+    const int32_t* mp = &S.map[S.boff[t]];
+    for (int64_t e = 0; e < n; ++e) {
+      const int32_t r = mp[e];
+      if (r < 0 || seen[r]) continue;
+      int64_t q = e;
+      for (int k = 0; k < D; ++k) {   // in-term index, first index fastest
+        idx[(size_t)r * D + k] = (int32_t)(q % dims[k]);
+        q /= dims[k];
+      }
+      seen[r] = 1;
+      ++nseen;
+    }
+  }
+  if (nseen != S.nb_ba) { set_error("some packed rows occur in no term"); return SG4_ERR_STATE; }
+  return SG4_OK;
+}
+
 int sg4_setup_destroy(sg4_setup* s) {
   if (s) { delete s->s; delete s; }
   return SG4_OK;

@@ -12,7 +12,12 @@ Mirrors the control flow of sub_propagation_Davidson
   (MakeResidual_Davidson, :1230-1304) divided by E_j - E0(ib), with the
   reference's guard 1/(D + 1e-3) when |D| <= conv_resi (NewVec_type = 4,
   sub_NewVec_Davidson :1440-1460), then normalised and orthogonalised twice
-  against the basis (:1529-1547); vectors with norm^2 < 1e-10 are dropped;
+  against the basis (:1529-1547), here as a block: two passes of block
+  Gram-Schmidt (two GEMMs each) followed by an orthonormalisation inside the
+  block from the eigen-decomposition of its Gram matrix, dropping directions
+  whose squared norm is below 1e-10 (the reference's cut for a vector after
+  its Gram-Schmidt); the span, hence the Ritz values, is the same as with the
+  reference's vector-by-vector sweep;
 * when the basis would exceed max_WP it is collapsed onto the current Ritz
   vectors (restart).
 
@@ -36,20 +41,29 @@ def ho_zeroth_order_diagonal(basis, device):
     """E0(ib) = sum_k (i_k + 1/2) scaleQ-independent HO energies of the packed
     basis functions (the role of get_Ene0_AT_ib_FROM_Basis,
     SRC/sub_Basis/sub_module_basis_set_alloc.f90:3122, for HO primitive bases)."""
-    lev = basis.levels()
-    mp = basis.map()
-    nb = basis.nb_ba
-    e0 = np.full(nb, np.nan)
-    boff = 0
-    for t in range(basis.nterms):
-        dims = [basis.basis1d(k, int(lev[t, k]))["nb"] for k in range(basis.D)]
-        n = int(np.prod(dims))
-        idx = np.indices(dims[::-1]).reshape(basis.D, -1)[::-1]   # first index fastest
-        rows = mp[boff:boff + n]
-        ok = rows >= 0
-        e0[rows[ok]] = idx[:, ok].sum(0) + 0.5 * basis.D
-        boff += n
+    idx = basis.packed()
+    e0 = idx.sum(axis=1) + 0.5 * basis.D
     return torch.tensor(e0, device=device, dtype=torch.float64)
+
+
+def _gram(A, B, chunks=256):
+    """A @ B.T for row blocks A (m, n), B (p, n) with a long inner dimension
+    (n = nb_ba): split over `chunks` pieces of n with one batched product plus
+    the remainder.  For small m, p and n ~ 1e6 the library GEMM picks
+    single-tile kernels without a K split that run for ~0.2 s."""
+    m, n = A.shape
+    p = B.shape[0]
+    k = n // chunks
+    n0 = k * chunks
+    out = None
+    if k > 0:
+        Ac = A[:, :n0].reshape(m, chunks, k).transpose(0, 1)
+        Bc = B[:, :n0].reshape(p, chunks, k).permute(1, 2, 0).contiguous()
+        out = torch.bmm(Ac, Bc).sum(0)
+    if n0 < n:
+        r = A[:, n0:] @ B[:, n0:].T
+        out = r if out is None else out + r
+    return out
 
 
 class DavidsonResult:
@@ -97,23 +111,17 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
         return out
 
     def orthonormalize(W, ndim):
-        keep = []
-        for j in range(W.shape[0]):
-            w = W[j]
-            w = w / torch.linalg.norm(w)
-            for _ in range(2):   # double Gram-Schmidt against the basis and accepted new vectors
-                if ndim:
-                    w = w - Vb[:ndim].T @ (Vb[:ndim] @ w)
-                for q in keep:
-                    w = w - q * torch.dot(q, w)
-                nrm2 = torch.dot(w, w).item()
-                if nrm2 < 1e-10:
-                    w = None
-                    break
-                w = w / np.sqrt(nrm2)
-            if w is not None:
-                keep.append(w)
-        return torch.stack(keep) if keep else None
+        W = W / torch.linalg.norm(W, dim=1, keepdim=True).clamp_min(1e-300)
+        for _ in range(2):   # double Gram-Schmidt against the basis, then inside the block
+            if ndim:
+                W = W - _gram(W, Vb[:ndim]) @ Vb[:ndim]
+            lam, U = np.linalg.eigh(_gram(W, W).cpu().numpy())
+            ok = lam > 1e-10
+            if not ok.any():
+                return None
+            T = torch.tensor(U[:, ok] / np.sqrt(lam[ok]), device=dev)
+            W = T.T @ W
+        return W
 
     phase_prec = prec
     # the FP32 residual floor is ~ eps32 x (Smolyak roundoff amplification) x ||H||
@@ -139,7 +147,7 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
         Vb[ndim:ndim + m] = W
         HV[ndim:ndim + m] = apply(W, phase_prec)
         ndim += m
-        Hp = Vb[:ndim] @ HV[:ndim].T
+        Hp = _gram(Vb[:ndim], HV[:ndim])
         Hp = 0.5 * (Hp + Hp.T)
         ev, vec = np.linalg.eigh(Hp.cpu().numpy())
         vec_t = torch.tensor(vec, device=dev)
@@ -162,12 +170,13 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
             # FP64 phase: restart on the Ritz vectors with an FP64 H.psi
             n_low = n_hpsi
             keep = min(ndim, nb_WP + block)
-            Q, _ = torch.linalg.qr(X_all[:keep].T)
-            Vb[:keep] = Q.T
+            Q = orthonormalize(X_all[:keep].clone(), 0)
+            keep = Q.shape[0]
+            Vb[:keep] = Q
             HV[:keep] = apply(Vb[:keep].contiguous(), PREC_FP64)
             ndim = keep
             phase_prec = PREC_FP64
-            Hp = Vb[:ndim] @ HV[:ndim].T
+            Hp = _gram(Vb[:ndim], HV[:ndim])
             ev, vec = np.linalg.eigh(0.5 * (Hp + Hp.T).cpu().numpy())
             vec_t = torch.tensor(vec, device=dev)
             X_all = vec_t.T @ Vb[:ndim]

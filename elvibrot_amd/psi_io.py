@@ -31,39 +31,13 @@ import numpy as np
 
 
 def packed_indices(basis):
-    """(nb_ba, D) int32: 0-based mode indices of every packed row, decoded
-    from the term map (every packed row occurs in some term)."""
+    """(nb_ba, D) int32: 0-based mode indices of every packed row
+    (sg4_setup_packed); SG4Batch-like objects without it are not supported."""
     cached = getattr(basis, "_packed_idx", None)
-    if cached is not None:
-        return cached
-    D, lev, mp = basis.D, basis.levels(), basis.map()
-    nbtab = {}
-    out = np.full((basis.nb_ba, D), -1, dtype=np.int32)
-    seen = np.zeros(basis.nb_ba, dtype=bool)
-    off = 0
-    for t in range(basis.nterms):
-        dims = []
-        for k in range(D):
-            key = (k, int(lev[t, k]))
-            if key not in nbtab:
-                nbtab[key] = basis.basis1d(k, key[1])["nb"]
-            dims.append(nbtab[key])
-        n = int(np.prod(dims))
-        rows = mp[off:off + n]
-        off += n
-        new = (rows >= 0) & ~seen[np.maximum(rows, 0)]
-        if not new.any():
-            continue
-        e = np.nonzero(new)[0]
-        r = rows[e]
-        for k in range(D):                 # first index fastest
-            out[r, k] = e % dims[k]
-            e = e // dims[k]
-        seen[r] = True
-    if not seen.all():
-        raise ValueError("some packed rows occur in no term")
-    basis._packed_idx = out
-    return out
+    if cached is None:
+        cached = basis.packed()
+        basis._packed_idx = cached
+    return cached
 
 
 def _fmt_real(x):

@@ -113,6 +113,10 @@ int sg4_setup_term_grid(const sg4_setup *s, int64_t t, double *q);
 /* 1-D rule and basis of (mode k, level l): x, w (nq), d0, d1, d2 (nq x nb) */
 int sg4_setup_basis1d(const sg4_setup *s, int32_t k, int32_t l, int32_t *nq, int32_t *nb,
                       double *x, double *w, double *d0, double *d1, double *d2);
+/* 0-based mode indices of every packed row, idx[r*D + k] (r < nb_ba): the
+ * nD index of packed basis function r (Rec_ndim_index, SG type 4 branch,
+ * SRC/sub_Basis/sub_module_basis.f90:3055-3065, 0-based here) */
+int sg4_setup_packed(const sg4_setup *s, int32_t *idx);
 int sg4_setup_destroy(sg4_setup *s);
 
 /* ---- device plan ----------------------------------------------------------
