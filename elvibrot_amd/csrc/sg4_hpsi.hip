@@ -365,6 +365,7 @@ __device__ __forceinline__ void dispatch_group(const GCtx<T>& c, int shape) {
 // that the map loads of a batch, then its psi loads (or atomics), are all in
 // flight together instead of one dependent load pair per element.
 constexpr int GK = 8;
+constexpr int GKS = 16;   // split path: psi loads per thread in flight (map from LDS)
 
 // One term with every active mode square (nq = nb), through group passes:
 // gather (psi rows into X, V + vshift into Y) -> basis->grid groups (in
@@ -471,7 +472,9 @@ __device__ __forceinline__ void term_t1_grouped(const KArgs& a, const KMats<T>& 
 // potential, grid->basis products.  The grid->basis groups run in reverse
 // order so that the last basis->grid group, the potential and the first
 // grid->basis group are one register pass.
-// LDS: X (N nv values of the item) followed by Vs (the N potential values).
+// LDS: X (N nv values of the item), Vs (the N potential values), Ms (the N
+// map entries, so that the psi gathers and the scatter-adds only wait for
+// LDS, not for a global map load).
 // ---------------------------------------------------------------------------
 template <class T>
 struct SCtx {
@@ -556,27 +559,38 @@ __device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km
   const int nt = blockDim.x;
   T* X = lds;
   T* Vs = lds + NE;
-  for (int e0 = threadIdx.x; e0 < NE; e0 += GK * nt) {
+  int32_t* Ms = (int32_t*)(Vs + N);
+  for (int p0 = threadIdx.x; p0 < N; p0 += GK * nt) {
     int m[GK];
-    T v[GK], x[GK];
+    T v[GK];
 #pragma unroll
     for (int k = 0; k < GK; ++k) {
-      const int e = e0 + k * nt;
-      m[k] = (e < NE) ? map[e >> lg] : -1;
-      v[k] = (e < NE) ? V[e >> lg] : (T)0;
+      const int p = p0 + k * nt;
+      m[k] = (p < N) ? map[p] : -1;
+      v[k] = (p < N) ? V[p] : (T)0;
     }
 #pragma unroll
     for (int k = 0; k < GK; ++k) {
-      const int e = e0 + k * nt;
-      x[k] = (m[k] >= 0) ? (T)a.psi[(int64_t)m[k] * a.nvec + it.v0 + (e & (nv - 1))] : (T)0;
-    }
-#pragma unroll
-    for (int k = 0; k < GK; ++k) {
-      const int e = e0 + k * nt;
-      if (e < NE) {
-        X[e] = x[k];
-        if ((e & (nv - 1)) == 0) Vs[e >> lg] = v[k];
+      const int p = p0 + k * nt;
+      if (p < N) {
+        Ms[p] = m[k];
+        Vs[p] = v[k];
       }
+    }
+  }
+  __syncthreads();
+  for (int e0 = threadIdx.x; e0 < NE; e0 += GKS * nt) {
+    T x[GKS];
+#pragma unroll
+    for (int k = 0; k < GKS; ++k) {
+      const int e = e0 + k * nt;
+      const int m = (e < NE) ? Ms[e >> lg] : -1;
+      x[k] = (m >= 0) ? (T)a.psi[(int64_t)m * a.nvec + it.v0 + (e & (nv - 1))] : (T)0;
+    }
+#pragma unroll
+    for (int k = 0; k < GKS; ++k) {
+      const int e = e0 + k * nt;
+      if (e < NE) X[e] = x[k];
     }
   }
   __syncthreads();
@@ -628,18 +642,13 @@ __device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km
     dispatch_split<T, false>(c, Td->gshape[g]);
     __syncthreads();
   }
-  for (int e0 = threadIdx.x; e0 < NE; e0 += GK * nt) {
-    int m[GK];
+  for (int e0 = threadIdx.x; e0 < NE; e0 += GKS * nt) {
 #pragma unroll
-    for (int k = 0; k < GK; ++k) {
+    for (int k = 0; k < GKS; ++k) {
       const int e = e0 + k * nt;
-      m[k] = (e < NE) ? map[e >> lg] : -1;
-    }
-#pragma unroll
-    for (int k = 0; k < GK; ++k) {
-      const int e = e0 + k * nt;
-      if (m[k] >= 0)
-        atomicAdd(&a.hpsi[(int64_t)m[k] * a.nvec + it.v0 + (e & (nv - 1))], w * (double)X[e]);
+      const int m = (e < NE) ? Ms[e >> lg] : -1;
+      if (m >= 0)
+        atomicAdd(&a.hpsi[(int64_t)m * a.nvec + it.v0 + (e & (nv - 1))], w * (double)X[e]);
     }
   }
 }
@@ -1275,7 +1284,7 @@ static int build_kinetic(sg4_plan* P) {
   if (P->type_op != 1 || !P->grouping) return SG4_OK;
   for (const TermDev& T : P->h_terms) {
     if (T.nact > 0 && T.ngrp == 0) return SG4_OK;
-    if (16.0 * T.nq > (double)LDS_MAX) return SG4_OK;   // FP64: one vector + potential
+    if (20.0 * T.nq > (double)LDS_MAX) return SG4_OK;   // FP64: one vector + potential + map
   }
   const size_t nmd = P->h_md.size();
   std::vector<int64_t> toff(nmd);
@@ -1484,7 +1493,7 @@ static int build_terms(sg4_plan* P) {
 // XCDs), so the chunks of a term share one L2 for their gathers; terms are
 // given to the XCD with the least accumulated work.
 // Split path (split = true): an item needs (nv + 1) N values of LDS (the
-// chunk and the potential); a term that does not fit the budget with one
+// chunk and the potential) and N map entries; a term that does not fit the budget with one
 // vector goes to the second list, launched with its own LDS size.
 static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_force = 0,
                      int budget = LDS_SMALL, int xcd = 0, bool split = false) {
@@ -1502,10 +1511,11 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
     for (int64_t t = 0; t < P->nterms; ++t) {
       const TermDev& T = P->h_terms[t];
       const int64_t n = (T.nact == 0) ? 0 : T.nq;
-      const bool large = 2 * n * esize > budget;
+      const bool large = 2 * n * esize + 4 * n > budget;
       const int cap = large ? LDS_MAX : budget;
       int gmax = 0;
-      while ((1 << (gmax + 1)) <= nvec && ((1 << (gmax + 1)) + 1) * n * esize <= cap) ++gmax;
+      while ((1 << (gmax + 1)) <= nvec && ((1 << (gmax + 1)) + 1) * n * esize + 4 * n <= cap)
+        ++gmax;
       double work = 0;
       for (int q = 0; q < T.nact; ++q) work += P->h_md[T.mat[q]].nq;
       work = (work + 1.0) * (double)std::max<int64_t>(n, 1);
@@ -1524,7 +1534,7 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
           flops_main += (f + (double)std::max<int64_t>(n, 1)) * (1 << g);
         }
         int& l = large ? lds2 : lds;
-        l = std::max<int>(l, (int)(((1 << g) + 1) * n * esize));
+        l = std::max<int>(l, (int)(((1 << g) + 1) * n * esize + 4 * n));
         all.push_back(h);
         v0 += 1 << g;
       }
