@@ -365,7 +365,6 @@ __device__ __forceinline__ void dispatch_group(const GCtx<T>& c, int shape) {
 // that the map loads of a batch, then its psi loads (or atomics), are all in
 // flight together instead of one dependent load pair per element.
 constexpr int GK = 8;
-constexpr int GKS = 16;   // split path: psi loads per thread in flight (map from LDS)
 
 // One term with every active mode square (nq = nb), through group passes:
 // gather (psi rows into X, V + vshift into Y) -> basis->grid groups (in
@@ -482,12 +481,21 @@ struct SCtx {
   const T* Mt[3];  // fused pass: grid->basis matrices
   T* X;
   const T* Vs;
+  const int32_t* Ms;     // the term's map, staged in LDS
+  const double* psi;     // gather: slabs read from the packed block (group 0 only)
+  double* hpsi;          // scatter: slabs added into the packed result (group 0 only)
+  double w;
+  int nvec, v0;
+  int gather, scatter;
   int AV;
   float invAV;
   int nfib;
   int lgnv;
 };
 
+// A pass over group 0 (A = 1, AV = nv) may read its slabs straight from the
+// packed block (gather) and add them straight into the result (scatter):
+// element J of slab (b, av) is grid point P b + J, vector av.
 template <class T, int N0, int N1, int N2, bool FUSED>
 __device__ __forceinline__ void split_pass(const SCtx<T>& c) {
   constexpr int P = N0 * N1 * N2;
@@ -496,8 +504,16 @@ __device__ __forceinline__ void split_pass(const SCtx<T>& c) {
     const int av = f - b * c.AV;
     const int base = av + c.AV * P * b;
     T x[P];
+    if (c.gather) {
 #pragma unroll
-    for (int J = 0; J < P; ++J) x[J] = c.X[base + c.AV * J];
+      for (int J = 0; J < P; ++J) {
+        const int m = c.Ms[P * b + J];
+        x[J] = (m >= 0) ? (T)c.psi[(int64_t)m * c.nvec + c.v0 + av] : (T)0;
+      }
+    } else {
+#pragma unroll
+      for (int J = 0; J < P; ++J) x[J] = c.X[base + c.AV * J];
+    }
     {
       cptr<T> M0 = (cptr<T>)uniform_ptr(c.M[0]);
       cptr<T> M1 = (cptr<T>)uniform_ptr(c.M[1]);
@@ -518,8 +534,16 @@ __device__ __forceinline__ void split_pass(const SCtx<T>& c) {
       if constexpr (N1 > 1) slab_mode<T, N1, N0, P>(x, Q1);
       if constexpr (N2 > 1) slab_mode<T, N2, N0 * N1, P>(x, Q2);
     }
+    if (c.scatter) {
 #pragma unroll
-    for (int J = 0; J < P; ++J) c.X[base + c.AV * J] = x[J];
+      for (int J = 0; J < P; ++J) {
+        const int m = c.Ms[P * b + J];
+        if (m >= 0) atomicAdd(&c.hpsi[(int64_t)m * c.nvec + c.v0 + av], c.w * (double)x[J]);
+      }
+    } else {
+#pragma unroll
+      for (int J = 0; J < P; ++J) c.X[base + c.AV * J] = x[J];
+    }
   }
 }
 
@@ -534,8 +558,10 @@ __device__ __forceinline__ void dispatch_split(const SCtx<T>& c, int shape) {
 }
 
 // Potential part of one term for one chunk of vectors (every active mode
-// square and grouped): gather -> basis->grid groups 0..G-2 -> fused group
-// G-1 -> grid->basis groups G-2..0 -> weighted scatter-add.
+// square and grouped): stage map and V -> basis->grid groups 0..G-2 (group 0
+// gathering from the packed block) -> fused group G-1 -> grid->basis groups
+// G-2..0 (group 0 scatter-adding into the result).  A term with one group
+// does all of it in one register pass.
 template <class T>
 __device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km, const Item& it,
                                               T* lds) {
@@ -579,25 +605,16 @@ __device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km
     }
   }
   __syncthreads();
-  for (int e0 = threadIdx.x; e0 < NE; e0 += GKS * nt) {
-    T x[GKS];
-#pragma unroll
-    for (int k = 0; k < GKS; ++k) {
-      const int e = e0 + k * nt;
-      const int m = (e < NE) ? Ms[e >> lg] : -1;
-      x[k] = (m >= 0) ? (T)a.psi[(int64_t)m * a.nvec + it.v0 + (e & (nv - 1))] : (T)0;
-    }
-#pragma unroll
-    for (int k = 0; k < GKS; ++k) {
-      const int e = e0 + k * nt;
-      if (e < NE) X[e] = x[k];
-    }
-  }
-  __syncthreads();
   const int G = Td->ngrp;
   SCtx<T> c;
   c.X = X;
   c.Vs = Vs;
+  c.Ms = Ms;
+  c.psi = a.psi;
+  c.hpsi = a.hpsi;
+  c.w = w;
+  c.nvec = a.nvec;
+  c.v0 = it.v0;
   c.lgnv = lg;
   int Ag[MAXACT];
   {
@@ -629,27 +646,24 @@ __device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km
     c.invAV = 1.0f / (float)c.AV;
     c.nfib = NE / P;
   };
+  // group 0 gathers; the last grid->basis pass (group 0) scatter-adds
   for (int g = 0; g + 1 < G; ++g) {
     setup(g, false, false);
+    c.gather = (g == 0);
+    c.scatter = 0;
     dispatch_split<T, false>(c, Td->gshape[g]);
     __syncthreads();
   }
   setup(G - 1, true, false);
+  c.gather = (G == 1);
+  c.scatter = (G == 1);
   dispatch_split<T, true>(c, Td->gshape[G - 1]);
-  __syncthreads();
   for (int g = G - 2; g >= 0; --g) {
-    setup(g, false, true);
-    dispatch_split<T, false>(c, Td->gshape[g]);
     __syncthreads();
-  }
-  for (int e0 = threadIdx.x; e0 < NE; e0 += GKS * nt) {
-#pragma unroll
-    for (int k = 0; k < GKS; ++k) {
-      const int e = e0 + k * nt;
-      const int m = (e < NE) ? Ms[e >> lg] : -1;
-      if (m >= 0)
-        atomicAdd(&a.hpsi[(int64_t)m * a.nvec + it.v0 + (e & (nv - 1))], w * (double)X[e]);
-    }
+    setup(g, false, true);
+    c.gather = 0;
+    c.scatter = (g == 0);
+    dispatch_split<T, false>(c, Td->gshape[g]);
   }
 }
 
@@ -665,11 +679,13 @@ __device__ __forceinline__ void split_items(const KArgs& a, const KMats<T>& km) 
   }
 }
 // the main item list, and (separate names for profiling) the terms that need
-// more LDS than the budget
+// more LDS than the budget.  The FP32 kernels fit 168 VGPRs and may run 768
+// threads (three waves per SIMD at one workgroup per CU).
+constexpr int NT32 = 768;
 __global__ __launch_bounds__(NT) void k_hpsi_t1s_f64(KArgs a, KMats<double> km) { split_items(a, km); }
-__global__ __launch_bounds__(NT) void k_hpsi_t1s_f32(KArgs a, KMats<float> km) { split_items(a, km); }
+__global__ __launch_bounds__(NT32) void k_hpsi_t1s_f32(KArgs a, KMats<float> km) { split_items(a, km); }
 __global__ __launch_bounds__(NT) void k_hpsi_t1s_big_f64(KArgs a, KMats<double> km) { split_items(a, km); }
-__global__ __launch_bounds__(NT) void k_hpsi_t1s_big_f32(KArgs a, KMats<float> km) { split_items(a, km); }
+__global__ __launch_bounds__(NT32) void k_hpsi_t1s_big_f32(KArgs a, KMats<float> km) { split_items(a, km); }
 
 // Kinetic part of the split path: hpsi(r, v) = sum_e K(r, col_e) psi(col_e, v)
 // (CSR rows, the diagonal first); one thread per (row, vector), plain stores,
@@ -1726,7 +1742,7 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
     if (!split) HIP_TRY(hipEventRecord(P->ev0, P->stream));
   }
   const unsigned grid = (unsigned)((cfg.grid > 0 && cfg.grid < ip->nitems) ? cfg.grid : ip->nitems);
-  const dim3 blk(P->type_op == 10 ? 256 : cfg.nthreads);
+  const dim3 blk(P->type_op == 10 ? 256 : (split ? cfg.nthreads : std::min(cfg.nthreads, NT)));
   if (split) {
     // kinetic rows (initialise hpsi), then the potential part of the terms
     hipLaunchKernelGGL(k_kin_rows, dim3(grid_blocks(n)), dim3(256), 0, P->stream,
@@ -2230,7 +2246,9 @@ int sg4_plan_autotune(sg4_plan* p, int32_t nvec, int32_t prec, float* best_ms) {
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
       cus = prop.multiProcessorCount;
   }
-  const int threads[] = {256, 512};
+  const int threads3[] = {256, 512, NT32};
+  const std::vector<int> threads(threads3,
+                                 threads3 + ((use_split(p) && prec != SG4_PREC_FP64) ? 3 : 2));
   const int budgets[] = {32 * 1024, 48 * 1024, 64 * 1024, 80 * 1024, 112 * 1024, 160 * 1024};
   const int grids[] = {0, 4, 8};   // 0: one per item; k: k workgroups per CU, persistent
   for (int nt : threads)
@@ -2274,10 +2292,11 @@ int sg4_plan_config(const sg4_plan* p, int32_t nvec, int32_t prec, int32_t* cfg4
 
 int sg4_plan_set_config(sg4_plan* p, int32_t nvec, int32_t prec, const int32_t* cfg4) {
   if (!p || !cfg4) { set_error("null argument"); return SG4_ERR_ARG; }
-  if ((cfg4[0] != 256 && cfg4[0] != 512) || cfg4[1] < 1024 || cfg4[1] > LDS_MAX || cfg4[2] < 0 ||
-      (cfg4[3] != 0 && cfg4[3] != 1)) {
-    set_error("launch configuration out of range (threads 256/512, 1 KiB <= LDS budget <= "
-              "160 KiB, grid >= 0, xcd 0/1)");
+  const bool t768 = cfg4[0] == NT32 && use_split(p) && prec != SG4_PREC_FP64;
+  if ((cfg4[0] != 256 && cfg4[0] != 512 && !t768) || cfg4[1] < 1024 || cfg4[1] > LDS_MAX ||
+      cfg4[2] < 0 || (cfg4[3] != 0 && cfg4[3] != 1)) {
+    set_error("launch configuration out of range (threads 256/512, or 768 for the FP32 split "
+              "path; 1 KiB <= LDS budget <= 160 KiB, grid >= 0, xcd 0/1)");
     return SG4_ERR_ARG;
   }
   LaunchCfg c;

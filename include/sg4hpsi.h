@@ -171,7 +171,7 @@ int sg4_plan_set_field(sg4_plan *p, const double *mu, double eps);
 int sg4_grad_V(sg4_plan *p, int32_t nvec, const double *phi, const double *psi, double *grad);
 
 /* Autotuner: times the H.psi kernel for blocks of nvec vectors over its
- * launch parameters (256/512 threads, LDS budget per workgroup, one workgroup
+ * launch parameters (256/512 threads, 768 for the FP32 split path; LDS budget per workgroup, one workgroup
  * per work item or a persistent grid, XCD-grouped item order) and keeps the
  * fastest for later applies; best_ms receives its time.  Allocates two
  * temporary blocks.  The reference's only tuner scans OpenMP thread counts
@@ -180,7 +180,8 @@ int sg4_plan_autotune(sg4_plan *p, int32_t nvec, int32_t prec, float *best_ms);
 /* the launch parameters in use for (nvec, prec): threads, LDS budget, grid, xcd */
 int sg4_plan_config(const sg4_plan *p, int32_t nvec, int32_t prec, int32_t *cfg4);
 /* set the launch parameters for (nvec, prec) (e.g. a configuration saved from
- * an earlier autotune): threads 256 or 512, LDS budget in bytes (<= 160 KiB),
+ * an earlier autotune): threads 256 or 512 (768 for the FP32 mode on the split
+ * path), LDS budget in bytes (<= 160 KiB),
  * grid (0 = one workgroup per item, else the persistent grid size), xcd 0/1.
  * No counterpart in the reference. */
 int sg4_plan_set_config(sg4_plan *p, int32_t nvec, int32_t prec, const int32_t *cfg4);
