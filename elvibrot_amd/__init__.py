@@ -116,7 +116,10 @@ class SG4Operator:
     potential:
       * ``("henon_heiles", lam)`` -- V evaluated on the device grids with
         kinetic -1/2 sum d^2 (type 1);
-      * a numpy array ``V`` of length ``basis.sum_nq`` with ``ck`` (type 1).
+      * ``("jacobi_triatomic", (mu_R, mu_r, D, a, Re, re, c))`` -- the type-10
+        triatomic model, V, G, Jac evaluated on the device grids;
+      * a numpy array ``V`` of length ``basis.sum_nq`` with ``ck`` (type 1) or
+        with ``G``, ``Jac``, ``sqRJ`` (type 10).
     """
 
     def __init__(self, basis, max_nvec=16, potential=("henon_heiles", 0.111803), ck=None,
@@ -131,6 +134,11 @@ class SG4Operator:
             _capi.check(lib().sg4_plan_set_stream(h, C.c_void_p(stream)))
         if isinstance(potential, tuple) and potential[0] == "henon_heiles":
             _capi.check(lib().sg4_plan_set_henon_heiles(h, float(potential[1])))
+            return
+        if isinstance(potential, tuple) and potential[0] == "jacobi_triatomic":
+            prm = (C.c_double * 7)(*[float(x) for x in potential[1]])
+            _capi.check(lib().sg4_plan_set_jacobi_triatomic(h, prm))
+            self.type_op = 10
             return
         c64 = lambda a: np.ascontiguousarray(a, dtype=np.float64)
         V = c64(potential)

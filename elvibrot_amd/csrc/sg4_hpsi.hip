@@ -1133,6 +1133,52 @@ __global__ void k_grid_hh(HHArgs a) {
   }
 }
 
+// Triatomic model in Jacobi coordinates (R, r, theta), J = 0, type-10
+// operator: V = D(1 - e^{-a(R-Re)})^2 + D(1 - e^{-a(r-re)})^2 + c cos^2 theta,
+// G = diag(1/mu_R, 1/mu_r, 1/(mu_R R^2) + 1/(mu_r r^2)), Jac = sin theta,
+// sqRhoOVERJac = 1, on every term grid (first index fastest; modes 0, 1, 2 =
+// R, r, theta).  The first-call evaluation of get_OpGrid_type10_OF_ONEDP_FOR_SG4
+// (SRC/sub_Operator/sub_OpPsi_SG4.f90:2536-2814) for this model.
+struct JacArgs {
+  const int32_t* levels;
+  const int64_t* qoff;
+  const double* nodes;
+  const int64_t* node_off;
+  const int32_t* nqtab;
+  int Lmax;
+  int64_t nterms;
+  double muR, mur, D, a, Re, re, c;
+  double *V, *G, *Jac, *sqRJ;
+};
+__global__ void k_grid_jacobi(JacArgs a) {
+  for (int64_t t = blockIdx.x; t < a.nterms; t += gridDim.x) {
+    const int64_t q0 = a.qoff[t], nqt = a.qoff[t + 1] - q0;
+    for (int64_t e = threadIdx.x; e < nqt; e += blockDim.x) {
+      int64_t r = e;
+      double q[3];
+      for (int k = 0; k < 3; ++k) {
+        const int l = a.levels[t * 3 + k];
+        const int idx = k * (a.Lmax + 1) + l;
+        const int n = a.nqtab[idx];
+        q[k] = a.nodes[a.node_off[idx] + (int)(r % n)];
+        r /= n;
+      }
+      const double mR = 1.0 - exp(-a.a * (q[0] - a.Re));
+      const double mr = 1.0 - exp(-a.a * (q[1] - a.re));
+      const double ct = cos(q[2]);
+      const int64_t p = q0 + e;
+      a.V[p] = a.D * mR * mR + a.D * mr * mr + a.c * ct * ct;
+      double* g = a.G + 9 * p;
+      for (int i = 0; i < 9; ++i) g[i] = 0.0;
+      g[0] = 1.0 / a.muR;
+      g[4] = 1.0 / a.mur;
+      g[8] = 1.0 / (a.muR * q[0] * q[0]) + 1.0 / (a.mur * q[1] * q[1]);
+      a.Jac[p] = sin(q[2]);
+      a.sqRJ[p] = 1.0;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // plan
 // ---------------------------------------------------------------------------
@@ -2115,6 +2161,46 @@ int sg4_plan_set_henon_heiles(sg4_plan* p, double lambda) {
   int rc;
   if ((rc = p->d_ck.alloc(sizeof(double) * p->nck))) return rc;
   HIP_TRY(hipMemcpy(p->d_ck.p, p->h_ck.data(), sizeof(double) * p->nck, hipMemcpyHostToDevice));
+  p->items.clear();
+  p->f32_valid = false;
+  rc = build_terms(p);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  return SG4_OK;
+}
+
+int sg4_plan_set_jacobi_triatomic(sg4_plan* p, const double* prm) {
+  if (!p || !prm) { set_error("null argument"); return SG4_ERR_ARG; }
+  if (!p->have_nodes) { set_error("plan was created without grid nodes"); return SG4_ERR_STATE; }
+  if (p->inst.size() != 1 || p->D != 3) {
+    set_error("the Jacobi triatomic model needs a single 3-mode instance (R, r, theta)");
+    return SG4_ERR_UNSUPPORTED;
+  }
+  if (!(prm[0] > 0) || !(prm[1] > 0)) { set_error("masses must be positive"); return SG4_ERR_ARG; }
+  int rc;
+  const size_t n = (size_t)p->sum_nq;
+  if ((rc = p->d_G.alloc(sizeof(double) * n * 9))) return rc;
+  if ((rc = p->d_Jac.alloc(sizeof(double) * n))) return rc;
+  if ((rc = p->d_sqRJ.alloc(sizeof(double) * n))) return rc;
+  const sg4_plan::Inst& I = p->inst[0];
+  JacArgs a;
+  a.levels = p->d_levels.as<int32_t>();
+  a.qoff = p->d_qoff.as<int64_t>();
+  a.nodes = p->d_nodes.as<double>();
+  a.node_off = p->d_node_off.as<int64_t>() + I.md0;
+  a.nqtab = p->d_nqtab.as<int32_t>() + I.md0;
+  a.Lmax = I.Lmax;
+  a.nterms = I.nterms;
+  a.muR = prm[0]; a.mur = prm[1]; a.D = prm[2]; a.a = prm[3]; a.Re = prm[4]; a.re = prm[5];
+  a.c = prm[6];
+  a.V = p->d_V.as<double>(); a.G = p->d_G.as<double>(); a.Jac = p->d_Jac.as<double>();
+  a.sqRJ = p->d_sqRJ.as<double>();
+  hipLaunchKernelGGL(k_grid_jacobi, dim3((unsigned)std::min<int64_t>(I.nterms, 65535)), dim3(256), 0,
+                     p->stream, a);
+  HIP_TRY(hipGetLastError());
+  p->type_op = 10;
+  (void)p->d_V0.alloc(0);
+  p->h_ck.assign(p->nck, 0.0);
   p->items.clear();
   p->f32_valid = false;
   rc = build_terms(p);

@@ -3,7 +3,8 @@ FOR_SG4 evaluates on the first call, SRC/sub_Operator/sub_OpPsi_SG4.f90
 :2536-3082).
 
 * Henon-Heiles: evaluated on the device (SG4Operator(..., ("henon_heiles", lam))).
-* Triatomic in Jacobi coordinates (R, r, theta), J = 0, type-10 operator:
+* Triatomic in Jacobi coordinates (R, r, theta), J = 0, type-10 operator
+  (evaluated on the device by default, on the host by jacobi_triatomic_grids):
   G = diag(1/mu_R, 1/mu_r, 1/(mu_R R^2) + 1/(mu_r r^2)), Jac = sin(theta),
   sqRhoOVERJac = 1 (the basis is orthonormal for dR dr sin(theta) dtheta), and
   V = Morse(R) + Morse(r) + c cos^2(theta).
@@ -51,8 +52,14 @@ def jacobi_triatomic_grids(basis, mu_R=AMU, mu_r=AMU, D=0.2, a=1.0, Re=3.0, re=2
     return V, G.reshape(-1), Jac, sqRJ
 
 
-def jacobi_triatomic_operator(L=10, max_nvec=8, **kw):
-    b = jacobi_triatomic_basis(L, **{k: v for k, v in kw.items() if k in
-                                      ("mu_R", "mu_r", "D", "a", "Re", "re")})
-    V, G, Jac, sqRJ = jacobi_triatomic_grids(b, **kw)
+def jacobi_triatomic_operator(L=10, max_nvec=8, device_grids=True, mu_R=AMU, mu_r=AMU, D=0.2,
+                              a=1.0, Re=3.0, re=2.0, coupling=0.01):
+    """The type-10 operator; its grids are evaluated on the device
+    (sg4_plan_set_jacobi_triatomic) or, with device_grids=False, on the host
+    (jacobi_triatomic_grids) and uploaded."""
+    b = jacobi_triatomic_basis(L, mu_R=mu_R, mu_r=mu_r, D=D, a=a, Re=Re, re=re)
+    if device_grids:
+        return SG4Operator(b, max_nvec=max_nvec,
+                           potential=("jacobi_triatomic", (mu_R, mu_r, D, a, Re, re, coupling)))
+    V, G, Jac, sqRJ = jacobi_triatomic_grids(b, mu_R, mu_r, D, a, Re, re, coupling)
     return SG4Operator(b, max_nvec=max_nvec, potential=V, type_op=10, G=G, Jac=Jac, sqRJ=sqRJ)

@@ -141,6 +141,14 @@ int sg4_plan_set_operator(sg4_plan *p, const sg4_operator *op);
  * calcN_op of Working_tests/MPI_tests/6D_Davidson_S0/sub_system_HenonHeiles.f:14-43),
  * kinetic -1/2 sum d^2 (type 1).  Needs tables->nodes at plan creation. */
 int sg4_plan_set_henon_heiles(sg4_plan *p, double lambda);
+/* Type-10 operator of a triatomic in Jacobi coordinates (modes R, r, theta),
+ * J = 0, evaluated on the device on every term grid (the first-call grid
+ * evaluation of get_OpGrid_type10_OF_ONEDP_FOR_SG4,
+ * SRC/sub_Operator/sub_OpPsi_SG4.f90:2536-2814, for this model):
+ * prm = {mu_R, mu_r, D, a, Re, re, c}; V = D(1-e^{-a(R-Re)})^2
+ * + D(1-e^{-a(r-re)})^2 + c cos^2(theta), G = diag(1/mu_R, 1/mu_r,
+ * 1/(mu_R R^2) + 1/(mu_r r^2)), Jac = sin(theta), sqRhoOVERJac = 1. */
+int sg4_plan_set_jacobi_triatomic(sg4_plan *p, const double *prm);
 
 /* Block H.psi: sub_TabOpPsi_FOR_SGtype4(Psi, OpPsi, para_Op)
  * (SRC/sub_Operator/sub_OpPsi_SG4.f90:681-702).  psi and hpsi are DEVICE

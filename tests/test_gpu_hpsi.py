@@ -261,3 +261,13 @@ def test_split_path_not_taken_for_rectangular_bases(cuda):
     assert not op.split_info()[0]
     psi = np.random.default_rng(6).standard_normal((b.nb_ba, 3))
     assert rel(op.apply_host(psi), o.hpsi(psi, V, -0.5 * np.ones(3))) < TOL
+
+
+def test_jacobi_device_grids_equal_host_grids(cuda):
+    """V, G, Jac evaluated on the device (sg4_plan_set_jacobi_triatomic) give
+    the same type-10 operator as the host-evaluated grids."""
+    import elvibrot_amd.potentials as P
+    op_dev = P.jacobi_triatomic_operator(6, max_nvec=4)
+    op_host = P.jacobi_triatomic_operator(6, max_nvec=4, device_grids=False)
+    psi = np.random.default_rng(8).standard_normal((op_dev.basis.nb_ba, 3))
+    assert rel(op_dev.apply_host(psi), op_host.apply_host(psi)) < 1e-14
