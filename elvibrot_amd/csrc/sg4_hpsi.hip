@@ -1217,7 +1217,8 @@ struct LaunchCfg {
   int nthreads = 256;          // 256 or 512
   int lds_budget = LDS_SMALL;  // bytes of work buffers per workgroup
   int grid = 0;                // 0: one workgroup per item, else persistent grid
-  int xcd = 0;                 // 1: keep the items of a term on one XCD
+  int xcd = 0;                 // 1: keep the items of a term on one XCD; k >= 2: keep a
+                               //    class of terms with equal levels in modes k..D-1 on one XCD
 };
 
 }  // namespace sg4
@@ -1553,14 +1554,16 @@ static int build_terms(sg4_plan* P) {
 // ones start first.  With xcd = 1 the items of one term are placed at
 // positions congruent modulo 8 (workgroups are dealt round-robin to the eight
 // XCDs), so the chunks of a term share one L2 for their gathers; terms are
-// given to the XCD with the least accumulated work.
+// given to the XCD with the least accumulated work.  With xcd = k >= 2 the unit kept on one XCD is a
+// class of terms with equal levels in modes k..D-1 (they share many rows), its
+// items consecutive in the lane.
 // Split path (split = true): an item needs (nv + 1) N values of LDS (the
 // chunk and the potential) and N map entries; a term that does not fit the budget with one
 // vector goes to the second list, launched with its own LDS size.
 static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_force = 0,
                      int budget = LDS_SMALL, int xcd = 0, bool split = false) {
   const int64_t key = (((((int64_t)nvec * 16 + esize + (nbuf_force ? 1 : 0)) * 1024 +
-                         budget / 1024) << 1) | (xcd ? 1 : 0)) * 2 + (split ? 1 : 0);
+                         budget / 1024) << 2) | (int64_t)xcd) * 2 + (split ? 1 : 0);
   auto itp = P->items.find(key);
   if (itp != P->items.end()) { *out = itp->second.get(); return SG4_OK; }
   struct HI { Item it; double cost; bool large; };
@@ -1649,19 +1652,53 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
     }
     std::vector<std::vector<Item>> lane(8);
     double acc[8] = {0};
-    std::map<int32_t, int> owner;
-    for (auto& x : all) {
-      if (x.large != large) continue;
-      auto o = owner.find(x.it.term);
-      int l;
-      if (o == owner.end()) {
-        l = (int)(std::min_element(acc, acc + 8) - acc);
-        owner[x.it.term] = l;
-      } else {
-        l = o->second;
+    if (xcd >= 2) {
+      // terms with equal levels in modes xcd..D-1 share the rows that agree in
+      // those modes: put such a class on one XCD, its items next to each other
+      // (classes by decreasing largest item, items by decreasing cost)
+      std::map<std::vector<int32_t>, int> cls_of;
+      std::vector<std::vector<const HI*>> cls;
+      for (auto& x : all) {
+        if (x.large != large) continue;
+        int64_t t = x.it.term;
+        size_t ii = 0;
+        while (ii + 1 < P->inst.size() && t >= P->inst[ii + 1].term0) ++ii;
+        const sg4_plan::Inst& I = P->inst[ii];
+        std::vector<int32_t> key(1, (int32_t)ii);
+        for (int k = xcd; k < I.D; ++k) key.push_back(P->h_levels[I.lev0 + (t - I.term0) * I.D + k]);
+        auto f = cls_of.find(key);
+        int c;
+        if (f == cls_of.end()) {
+          c = (int)cls.size();
+          cls_of[key] = c;
+          cls.emplace_back();
+        } else {
+          c = f->second;
+        }
+        cls[c].push_back(&x);
       }
-      lane[l].push_back(x.it);
-      acc[l] += x.cost;
+      for (auto& members : cls) {
+        double w = 0;
+        for (const HI* x : members) w += x->cost;
+        const int l = (int)(std::min_element(acc, acc + 8) - acc);
+        for (const HI* x : members) lane[l].push_back(x->it);
+        acc[l] += w;
+      }
+    } else {
+      std::map<int32_t, int> owner;
+      for (auto& x : all) {
+        if (x.large != large) continue;
+        auto o = owner.find(x.it.term);
+        int l;
+        if (o == owner.end()) {
+          l = (int)(std::min_element(acc, acc + 8) - acc);
+          owner[x.it.term] = l;
+        } else {
+          l = o->second;
+        }
+        lane[l].push_back(x.it);
+        acc[l] += x.cost;
+      }
     }
     size_t m = 0;
     for (auto& v : lane) m = std::max(m, v.size());
@@ -2340,7 +2377,7 @@ int sg4_plan_autotune(sg4_plan* p, int32_t nvec, int32_t prec, float* best_ms) {
   for (int nt : threads)
     for (int bu : budgets)
       for (int gk : grids)
-        for (int xc = 0; xc < 2; ++xc) {
+        for (int xc = 0; xc < 5; ++xc) {
           LaunchCfg c;
           c.nthreads = nt; c.lds_budget = bu; c.grid = gk ? gk * cus : 0; c.xcd = xc;
           ItemPlan* ip = nullptr;
@@ -2380,9 +2417,9 @@ int sg4_plan_set_config(sg4_plan* p, int32_t nvec, int32_t prec, const int32_t* 
   if (!p || !cfg4) { set_error("null argument"); return SG4_ERR_ARG; }
   const bool t768 = cfg4[0] == NT32 && use_split(p) && prec != SG4_PREC_FP64;
   if ((cfg4[0] != 256 && cfg4[0] != 512 && !t768) || cfg4[1] < 1024 || cfg4[1] > LDS_MAX ||
-      cfg4[2] < 0 || (cfg4[3] != 0 && cfg4[3] != 1)) {
+      cfg4[2] < 0 || cfg4[3] < 0 || cfg4[3] > 8) {
     set_error("launch configuration out of range (threads 256/512, or 768 for the FP32 split "
-              "path; 1 KiB <= LDS budget <= 160 KiB, grid >= 0, xcd 0/1)");
+              "path; 1 KiB <= LDS budget <= 160 KiB, grid >= 0, 0 <= xcd <= 8)");
     return SG4_ERR_ARG;
   }
   LaunchCfg c;

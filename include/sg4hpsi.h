@@ -190,7 +190,9 @@ int sg4_plan_config(const sg4_plan *p, int32_t nvec, int32_t prec, int32_t *cfg4
 /* set the launch parameters for (nvec, prec) (e.g. a configuration saved from
  * an earlier autotune): threads 256 or 512 (768 for the FP32 mode on the split
  * path), LDS budget in bytes (<= 160 KiB),
- * grid (0 = one workgroup per item, else the persistent grid size), xcd 0/1.
+ * grid (0 = one workgroup per item, else the persistent grid size), xcd 0..8
+ * (item order: 0 plain, 1 a term's items on one XCD, k >= 2 a class of terms
+ * with equal levels in modes k..D-1 on one XCD).
  * No counterpart in the reference. */
 int sg4_plan_set_config(sg4_plan *p, int32_t nvec, int32_t prec, const int32_t *cfg4);
 /* sg4_apply_rows through a captured HIP graph (zeroing + H.psi kernel), one
