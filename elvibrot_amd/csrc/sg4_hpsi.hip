@@ -1562,8 +1562,9 @@ static int build_terms(sg4_plan* P) {
 // vector goes to the second list, launched with its own LDS size.
 static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_force = 0,
                      int budget = LDS_SMALL, int xcd = 0, bool split = false) {
-  const int64_t key = (((((int64_t)nvec * 16 + esize + (nbuf_force ? 1 : 0)) * 1024 +
-                         budget / 1024) << 2) | (int64_t)xcd) * 2 + (split ? 1 : 0);
+  const int64_t key =
+      ((((int64_t)nvec * 16 + esize + (nbuf_force ? 1 : 0)) * 1024 + budget / 1024) * 32 + xcd) * 2 +
+      (split ? 1 : 0);
   auto itp = P->items.find(key);
   if (itp != P->items.end()) { *out = itp->second.get(); return SG4_OK; }
   struct HI { Item it; double cost; bool large; };
