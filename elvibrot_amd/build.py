@@ -24,16 +24,39 @@ def hipcc():
             return c
 
 
+SOURCES = ("sg4_setup.cpp", "sg4_hpsi.hip", "sg4_k_split_f64.hip", "sg4_k_split_f32.hip",
+           "sg4_k_grid_f64.hip", "sg4_k_grid_f32.hip")
+
+
 def build(verbose=False):
-    srcs = [os.path.join(CSRC, f) for f in ("sg4_setup.cpp", "sg4_hpsi.hip")]
+    """Compile the translation units in parallel into _build/, then link."""
+    obj_dir = os.path.join(HERE, "_build", "obj")
+    os.makedirs(obj_dir, exist_ok=True)
+    procs = []
+    for f in SOURCES:
+        obj = os.path.join(obj_dir, f + ".o")
+        cmd = [hipcc()] + HIPCC_FLAGS + ["-c", "-o", obj, os.path.join(CSRC, f)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((f, obj, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                               stderr=subprocess.STDOUT, text=True)))
+    objs, failed = [], []
+    for f, obj, pr in procs:
+        out, _ = pr.communicate()
+        if pr.returncode != 0:
+            failed.append(f)
+            sys.stderr.write(out)
+        objs.append(obj)
+    if failed:
+        raise RuntimeError("hipcc failed on " + ", ".join(failed))
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-shared", "-o", tmp] + srcs
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-shared", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libsg4hpsi.so")
+        raise RuntimeError("hipcc failed linking libsg4hpsi.so")
     os.replace(tmp, LIB)
     return LIB
 

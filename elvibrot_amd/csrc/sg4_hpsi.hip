@@ -6,15 +6,18 @@
 // each term's coefficients, transforms basis->grid mode by mode, applies the
 // potential and the kinetic derivatives on the grid, transforms back and
 // scatter-adds W_t times the result.  Here one workgroup handles one
-// (term, chunk of vectors) work item entirely in LDS:
-//   pass BG0   : gather from the packed block fused with the first mode product
-//   passes BG  : remaining basis->grid mode products (LDS -> LDS)
-//   passes GR  : V psi + sum_k c_k d2_k psi on the grid
-//   passes GB  : grid->basis mode products, the last one fused with the
-//                weighted scatter-add into the packed result
-// Each mode product is done fibre by fibre: one thread loads a fibre into
-// registers and multiplies it by the small 1-D matrix, whose entries are
-// wave-uniform.
+// (term, chunk of vectors) work item in LDS, in passes over groups of up to
+// three modes whose slabs a thread holds in registers while it multiplies
+// them by the small 1-D matrices (entries wave-uniform, read through the
+// scalar cache).  Three type-1 paths:
+//   split path (k_kin_rows + k_hpsi_t1s_*): the kinetic part as one sparse
+//     matrix over the packed rows, the terms keep only basis->grid, the
+//     potential and grid->basis, with the gather and the weighted scatter-add
+//     folded into the first and last passes;
+//   grouped grid path (k_hpsi_t1_*, term_t1_grouped): the reference's
+//     structure with the kinetic derivatives on the grid;
+//   per-mode grid path (k_hpsi_t1_*, term_t1): one pass per mode, any sizes.
+// Type 10 (d1 G d1): k_hpsi_t10_f64.  Eigenvalue gradients: k_grad_v.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,8 +32,7 @@
 #include <string>
 #include <vector>
 
-#include "sg4_internal.h"
-#include "sg4_shapes.h"
+#include "sg4_device.h"
 
 namespace sg4 {
 
@@ -45,994 +47,6 @@ void set_error(const std::string& msg) { g_last_error = msg; }
       return SG4_ERR_HIP;                                                         \
     }                                                                             \
   } while (0)
-
-constexpr int MAXACT = 16;
-constexpr int NT = 512;                 // maximum threads per workgroup (launch bound)
-constexpr int LDS_SMALL = 64 * 1024;    // LDS budget of the common class (2 WG/CU)
-constexpr int LDS_MAX = 160 * 1024;     // gfx950 LDS per workgroup
-
-struct MatDev {          // offsets (in doubles) of the 1-D matrices of (k,l)
-  int64_t B, BtW, D1, D2;
-  int32_t nq, nb;
-  double b00, btw00, d200;  // entries used when nq = nb = 1
-};
-
-struct TermDev {
-  int64_t boff, qoff;
-  double wgt;      // W_t times the 1x1 transforms of the inactive modes
-  double vshift;   // type 1: sum over inactive modes of c_k d2_k(0,0)
-  int32_t nb, nq;  // product sizes
-  int32_t nact, pad;
-  int16_t mat[MAXACT];   // MatDev index of each active mode
-  int16_t kmode[MAXACT]; // index of the mode's kinetic coefficient (ck)
-  // register-blocked passes: groups of consecutive active modes
-  int32_t ngrp;          // 0: use the one-mode-per-pass path
-  int8_t gfirst[MAXACT]; // first active mode of group g
-  int8_t gshape[MAXACT]; // shape id (sg4_shapes.h)
-};
-
-struct Item {
-  int32_t term;
-  int16_t v0;
-  int8_t lgnv;
-  int8_t big;     // second buffer in global scratch (slot) instead of LDS
-  int32_t slot;
-  int32_t pad;
-};
-
-struct KArgs {
-  const Item* items;
-  const TermDev* terms;
-  const MatDev* md;
-  const int32_t* map;
-  const double* psi;
-  double* hpsi;
-  double* scratch;
-  int64_t slot_size;
-  int nvec;
-  int nitems;
-};
-
-// ---------------------------------------------------------------------------
-// device helpers
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ int fdiv(int n, int d, float inv) {
-  int q = (int)((float)n * inv);
-  int r = n - q * d;
-  while (r < 0) { --q; r += d; }
-  while (r >= d) { ++q; r -= d; }
-  return q;
-}
-
-enum { IN_LDS = 0, IN_GATHER = 1 };
-enum { OUT_STORE = 0, OUT_ADD = 1, OUT_VSTORE = 2, OUT_SCATTER = 3 };
-
-template <class T>
-using cptr = const __attribute__((address_space(4))) T*;   // scalar (constant) loads
-
-// the value is the same in every lane (it derives from the work item);
-// readfirstlane states that so the pointer can live in scalar registers
-template <class T>
-__device__ __forceinline__ const T* uniform_ptr(const T* p) {
-  const uint64_t v = (uint64_t)p;
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return (const T*)(((uint64_t)hi << 32) | lo);
-}
-
-template <class T>
-struct PassCtx {
-  const T* MR;   // NO x NI, row-major: MR[i*NI + j]
-  const T* in;
-  T* out;
-  int AV;        // A * nv
-  float invAV;
-  int A;
-  int nfib;
-  int lgnv;
-  T coef;
-  // gather / scatter
-  const int32_t* __restrict__ map;  // term's map slice
-  const double* __restrict__ psi;
-  double* hpsi;
-  int nvec, v0;
-  double wgt;
-  // grid potential
-  const T* __restrict__ V;  // term's grid slice
-  T vshift;
-};
-
-__device__ __forceinline__ double fmat(double a, double b, double c) { return fma(a, b, c); }
-__device__ __forceinline__ float fmat(float a, float b, float c) { return fmaf(a, b, c); }
-
-template <class T, int KOUT>
-__device__ __forceinline__ void emit(const PassCtx<T>& c, int av, int b, int i, int NO, T y, T xi) {
-  if constexpr (KOUT == OUT_STORE) {
-    c.out[av + c.AV * (i + NO * b)] = y;
-  } else if constexpr (KOUT == OUT_ADD) {
-    T* o = c.out + av + c.AV * (i + NO * b);
-    *o = fmat(c.coef, y, *o);
-  } else if constexpr (KOUT == OUT_VSTORE) {
-    const int e = (av >> c.lgnv) + c.A * (i + NO * b);
-    c.out[av + c.AV * (i + NO * b)] = fmat(c.coef, y, (c.V[e] + c.vshift) * xi);
-  } else {
-    // last active mode: nothing after it, b == 0
-    const int e = (av >> c.lgnv) + c.A * i;
-    const int m = c.map[e];
-    if (m >= 0)
-      atomicAdd(&c.hpsi[(int64_t)m * c.nvec + c.v0 + (av & ((1 << c.lgnv) - 1))],
-                c.wgt * (double)y);
-  }
-}
-
-// One mode product over all fibres of the current tensor, one fibre per
-// thread: y(i) = sum_j M(i,j) x(j).  KIN selects where the fibre comes from
-// (LDS, or the packed block through the map), KOUT what is done with y.
-// The matrix rows are read with scalar loads (wave-uniform); the pointer is
-// made opaque in every iteration so that the rows are not hoisted into
-// vector registers across the fibre loop.
-template <class T, int NI, int NO, int KIN, int KOUT>
-__device__ __forceinline__ void mode_pass(const PassCtx<T>& c) {
-  for (int f = threadIdx.x; f < c.nfib; f += blockDim.x) {
-    const int b = fdiv(f, c.AV, c.invAV);
-    const int av = f - b * c.AV;
-    T x[NI];
-    if constexpr (KIN == IN_GATHER) {
-      // first active mode: A = 1 so av is the vector index
-      const int32_t* mp = c.map + NI * b;
-#pragma unroll
-      for (int j = 0; j < NI; ++j) {
-        const int m = mp[j];
-        x[j] = (m >= 0) ? (T)c.psi[(int64_t)m * c.nvec + c.v0 + av] : (T)0;
-      }
-    } else {
-      const T* src = c.in + av + c.AV * NI * b;
-#pragma unroll
-      for (int j = 0; j < NI; ++j) x[j] = src[c.AV * j];
-    }
-    cptr<T> M = (cptr<T>)uniform_ptr(c.MR);
-    asm volatile("" : "+s"(M));
-#pragma unroll
-    for (int i = 0; i < NO; ++i) {
-      T y = 0;
-#pragma unroll
-      for (int j = 0; j < NI; ++j) y = fmat(M[i * NI + j], x[j], y);
-      emit<T, KOUT>(c, av, b, i, NO, y, (KOUT == OUT_VSTORE) ? x[i < NI ? i : 0] : (T)0);
-    }
-  }
-}
-
-// Any sizes (rectangular, > 24, or the rare terms whose second buffer is in
-// global scratch): the fibre is re-read for every output instead of being
-// held in registers, which keeps this path's register use small.
-template <class T, int KIN, int KOUT>
-__device__ __noinline__ void mode_pass_generic(const PassCtx<T>& c, int NI, int NO) {
-  for (int f = threadIdx.x; f < c.nfib; f += blockDim.x) {
-    const int b = fdiv(f, c.AV, c.invAV);
-    const int av = f - b * c.AV;
-    for (int i = 0; i < NO; ++i) {
-      T y = 0, xi = 0;
-      for (int j = 0; j < NI; ++j) {
-        T xj;
-        if constexpr (KIN == IN_GATHER) {
-          const int m = c.map[NI * b + j];
-          xj = (m >= 0) ? (T)c.psi[(int64_t)m * c.nvec + c.v0 + av] : (T)0;
-        } else {
-          xj = c.in[av + c.AV * (j + NI * b)];
-        }
-        if (j == i) xi = xj;
-        y = fmat(c.MR[i * NI + j], xj, y);
-      }
-      emit<T, KOUT>(c, av, b, i, NO, y, xi);
-    }
-  }
-}
-
-template <class T, bool FAST, int KIN, int KOUT>
-__device__ __forceinline__ void dispatch_pass(const PassCtx<T>& c, int NI, int NO) {
-  if (FAST && NI == NO) {
-    switch (NI) {
-#define SG4_CASE(n) case n: mode_pass<T, n, n, KIN, KOUT>(c); return;
-      SG4_CASE(1) SG4_CASE(2) SG4_CASE(3) SG4_CASE(4) SG4_CASE(5) SG4_CASE(6) SG4_CASE(7)
-      SG4_CASE(8) SG4_CASE(9) SG4_CASE(10) SG4_CASE(11) SG4_CASE(12) SG4_CASE(13)
-      SG4_CASE(14) SG4_CASE(15) SG4_CASE(16) SG4_CASE(17) SG4_CASE(18) SG4_CASE(19)
-      SG4_CASE(20) SG4_CASE(21) SG4_CASE(22) SG4_CASE(23) SG4_CASE(24)
-#undef SG4_CASE
-      default: break;
-    }
-  }
-  mode_pass_generic<T, KIN, KOUT>(c, NI, NO);
-}
-
-// ---------------------------------------------------------------------------
-// Register-blocked group passes.  A slab = all grid values of a group of up to
-// three consecutive active modes (sizes N0, N1, N2) for fixed indices of the
-// other modes and one vector; one thread holds a slab in registers and applies
-// every 1-D matrix of the group before writing it back, so a term costs one
-// pass per group instead of one per mode.
-//   KIND_CHAIN: x <- M2 x3 (M1 x2 (M0 x1 x))          (basis<->grid, in place)
-//   KIND_GRID : y <- [V x | y] + sum_i c_i (D2_i xi x)  (potential + kinetic)
-// Slab element J (first index fastest) of slab f lives at base + AV*J with
-// base = (f mod AV) + AV*P*(f div AV), P = N0 N1 N2, AV = A*nv.
-// ---------------------------------------------------------------------------
-enum { KIND_CHAIN = 0, KIND_GRID = 1 };
-
-template <class T>
-struct KMats {
-  const T* mats;
-  const T* V;
-  const T* ck;
-};
-
-
-template <class T>
-struct GCtx {
-  const T* M[3];   // row-major n x n matrices of the group's modes
-  T c[3];          // kinetic coefficients (grid kind)
-  const T* in;
-  T* out;
-  int AV;
-  float invAV;
-  int A;
-  int nfib;
-  int first;       // grid kind: out holds V + vshift (staged by the gather); start from V x
-};
-
-// y = M x along one mode of the slab (stride S, N entries), in place
-template <class T, int N, int S, int P>
-__device__ __forceinline__ void slab_mode(T (&x)[P], cptr<T> M) {
-#pragma unroll
-  for (int o = 0; o < P / N; ++o) {
-    const int base = (o % S) + (o / S) * S * N;
-    T y[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      T acc = 0;
-#pragma unroll
-      for (int j = 0; j < N; ++j) acc = fmat(M[i * N + j], x[base + S * j], acc);
-      y[i] = acc;
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) x[base + S * i] = y[i];
-  }
-}
-
-// acc += c (M x) along one mode of the slab
-template <class T, int N, int S, int P>
-__device__ __forceinline__ void slab_mode_acc(const T (&x)[P], T (&acc)[P], cptr<T> M, T c) {
-#pragma unroll
-  for (int o = 0; o < P / N; ++o) {
-    const int base = (o % S) + (o / S) * S * N;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-      T y = 0;
-#pragma unroll
-      for (int j = 0; j < N; ++j) y = fmat(M[i * N + j], x[base + S * j], y);
-      acc[base + S * i] = fmat(c, y, acc[base + S * i]);
-    }
-  }
-}
-
-template <class T, int N0, int N1, int N2, int KIND>
-__device__ __forceinline__ void group_pass(const GCtx<T>& c) {
-  constexpr int P = N0 * N1 * N2;
-  for (int f = threadIdx.x; f < c.nfib; f += blockDim.x) {
-    const int b = fdiv(f, c.AV, c.invAV);
-    const int av = f - b * c.AV;
-    const int base = av + c.AV * P * b;
-    T x[P];
-#pragma unroll
-    for (int J = 0; J < P; ++J) x[J] = c.in[base + c.AV * J];
-    cptr<T> M0 = (cptr<T>)uniform_ptr(c.M[0]);
-    cptr<T> M1 = (cptr<T>)uniform_ptr(c.M[1]);
-    cptr<T> M2 = (cptr<T>)uniform_ptr(c.M[2]);
-    asm volatile("" : "+s"(M0), "+s"(M1), "+s"(M2));
-    if constexpr (KIND == KIND_CHAIN) {
-      slab_mode<T, N0, 1, P>(x, M0);
-      if constexpr (N1 > 1) slab_mode<T, N1, N0, P>(x, M1);
-      if constexpr (N2 > 1) slab_mode<T, N2, N0 * N1, P>(x, M2);
-#pragma unroll
-      for (int J = 0; J < P; ++J) c.out[base + c.AV * J] = x[J];
-    } else {
-      T acc[P];
-      if (c.first) {
-#pragma unroll
-        for (int J = 0; J < P; ++J) acc[J] = c.out[base + c.AV * J] * x[J];
-      } else {
-#pragma unroll
-        for (int J = 0; J < P; ++J) acc[J] = c.out[base + c.AV * J];
-      }
-      slab_mode_acc<T, N0, 1, P>(x, acc, M0, c.c[0]);
-      if constexpr (N1 > 1) slab_mode_acc<T, N1, N0, P>(x, acc, M1, c.c[1]);
-      if constexpr (N2 > 1) slab_mode_acc<T, N2, N0 * N1, P>(x, acc, M2, c.c[2]);
-#pragma unroll
-      for (int J = 0; J < P; ++J) c.out[base + c.AV * J] = acc[J];
-    }
-  }
-}
-
-template <class T, int KIND>
-__device__ __forceinline__ void dispatch_group(const GCtx<T>& c, int shape) {
-  switch (shape) {
-#define SG4_GCASE(id, a, b, d) case id: group_pass<T, a, b, d, KIND>(c); return;
-    SG4_SHAPES(SG4_GCASE)
-#undef SG4_GCASE
-    default: return;
-  }
-}
-
-// Gather and scatter walk the item in batches of GK elements per thread so
-// that the map loads of a batch, then its psi loads (or atomics), are all in
-// flight together instead of one dependent load pair per element.
-constexpr int GK = 8;
-
-// One term with every active mode square (nq = nb), through group passes:
-// gather (psi rows into X, V + vshift into Y) -> basis->grid groups (in
-// place on X) -> grid groups (X -> Y, the first one multiplying by the staged
-// potential) -> grid->basis groups (in place on Y) -> weighted scatter-add.
-template <class T>
-__device__ __forceinline__ void term_t1_grouped(const KArgs& a, const KMats<T>& km,
-                                                const Item& it, T* lds) {
-  const TermDev* Td = &a.terms[it.term];
-  const int lg = it.lgnv, nv = 1 << lg;
-  const int32_t* map = a.map + Td->boff;
-  const T* V = km.V + Td->qoff;
-  const T vshift = (T)Td->vshift;
-  const int N = Td->nq;
-  const int NE = N * nv;
-  const int nt = blockDim.x;
-  T* X = lds;
-  T* Y = lds + NE;
-  for (int e0 = threadIdx.x; e0 < NE; e0 += GK * nt) {
-    int m[GK];
-    T v[GK], x[GK];
-#pragma unroll
-    for (int k = 0; k < GK; ++k) {
-      const int e = e0 + k * nt;
-      m[k] = (e < NE) ? map[e >> lg] : -1;
-      v[k] = (e < NE) ? V[e >> lg] : (T)0;
-    }
-#pragma unroll
-    for (int k = 0; k < GK; ++k) {
-      const int e = e0 + k * nt;
-      x[k] = (m[k] >= 0) ? (T)a.psi[(int64_t)m[k] * a.nvec + it.v0 + (e & (nv - 1))] : (T)0;
-    }
-#pragma unroll
-    for (int k = 0; k < GK; ++k) {
-      const int e = e0 + k * nt;
-      if (e < NE) {
-        X[e] = x[k];
-        Y[e] = v[k] + vshift;
-      }
-    }
-  }
-  __syncthreads();
-  GCtx<T> c;
-  for (int stage = 0; stage < 3; ++stage) {
-    int A = 1;
-    for (int g = 0; g < Td->ngrp; ++g) {
-      const int p0 = Td->gfirst[g];
-      const int p1 = (g + 1 < Td->ngrp) ? Td->gfirst[g + 1] : Td->nact;
-      int P = 1;
-      for (int q = 0; q < 3; ++q) {
-        const int pm = p0 + q;
-        if (pm < p1) {
-          const MatDev* md = &a.md[Td->mat[pm]];
-          P *= md->nq;
-          c.M[q] = km.mats + (stage == 0 ? md->B : stage == 1 ? md->D2 : md->BtW);
-          c.c[q] = km.ck[Td->kmode[pm]];
-        } else {
-          c.M[q] = c.M[0];
-          c.c[q] = 0;
-        }
-      }
-      c.AV = A * nv;
-      c.invAV = 1.0f / (float)c.AV;
-      c.A = A;
-      c.nfib = NE / P;
-      if (stage == 1) {
-        c.in = X; c.out = Y; c.first = (g == 0);
-        dispatch_group<T, KIND_GRID>(c, Td->gshape[g]);
-      } else {
-        T* buf = (stage == 0) ? X : Y;
-        c.in = buf; c.out = buf; c.first = 0;
-        dispatch_group<T, KIND_CHAIN>(c, Td->gshape[g]);
-      }
-      __syncthreads();
-      A *= P;
-    }
-  }
-  const double w = Td->wgt;
-  for (int e0 = threadIdx.x; e0 < NE; e0 += GK * nt) {
-    int m[GK];
-#pragma unroll
-    for (int k = 0; k < GK; ++k) {
-      const int e = e0 + k * nt;
-      m[k] = (e < NE) ? map[e >> lg] : -1;
-    }
-#pragma unroll
-    for (int k = 0; k < GK; ++k) {
-      const int e = e0 + k * nt;
-      if (m[k] >= 0)
-        atomicAdd(&a.hpsi[(int64_t)m[k] * a.nvec + it.v0 + (e & (nv - 1))], w * (double)Y[e]);
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Split type-1 path.  When the quadrature of every 1-D basis integrates the
-// products of two basis functions exactly (B^T W B = I, checked when the plan
-// is built), the kinetic part of a term needs no grid: along mode k it is
-// c_k T_k(l_k) with T_k(l) = BtW D2 B (nb x nb), the identity in the other
-// modes.  Summed over the Smolyak terms with the combination weights, the
-// kinetic part of H becomes one sparse matrix K between packed rows that
-// differ in one mode index (built on the host, applied by k_kin_rows), and a
-// term only needs  W_t P_t^T B^T W V_t B P_t :  basis->grid products, the
-// potential, grid->basis products.  The grid->basis groups run in reverse
-// order so that the last basis->grid group, the potential and the first
-// grid->basis group are one register pass.
-// LDS: X (N nv values of the item), Vs (the N potential values), Ms (the N
-// map entries, so that the psi gathers and the scatter-adds only wait for
-// LDS, not for a global map load).
-// ---------------------------------------------------------------------------
-template <class T>
-struct SCtx {
-  const T* M[3];   // basis->grid (or, not fused, the pass's) matrices of the group
-  const T* Mt[3];  // fused pass: grid->basis matrices
-  T* X;
-  const T* Vs;
-  const int32_t* Ms;     // the term's map, staged in LDS
-  const double* psi;     // gather: slabs read from the packed block (group 0 only)
-  double* hpsi;          // scatter: slabs added into the packed result (group 0 only)
-  double w;
-  int nvec, v0;
-  int gather, scatter;
-  int AV;
-  float invAV;
-  int nfib;
-  int lgnv;
-};
-
-// A pass over group 0 (A = 1, AV = nv) may read its slabs straight from the
-// packed block (gather) and add them straight into the result (scatter):
-// element J of slab (b, av) is grid point P b + J, vector av.
-template <class T, int N0, int N1, int N2, bool FUSED>
-__device__ __forceinline__ void split_pass(const SCtx<T>& c) {
-  constexpr int P = N0 * N1 * N2;
-  for (int f = threadIdx.x; f < c.nfib; f += blockDim.x) {
-    const int b = fdiv(f, c.AV, c.invAV);
-    const int av = f - b * c.AV;
-    const int base = av + c.AV * P * b;
-    T x[P];
-    if (c.gather) {
-#pragma unroll
-      for (int J = 0; J < P; ++J) {
-        const int m = c.Ms[P * b + J];
-        x[J] = (m >= 0) ? (T)c.psi[(int64_t)m * c.nvec + c.v0 + av] : (T)0;
-      }
-    } else {
-#pragma unroll
-      for (int J = 0; J < P; ++J) x[J] = c.X[base + c.AV * J];
-    }
-    {
-      cptr<T> M0 = (cptr<T>)uniform_ptr(c.M[0]);
-      cptr<T> M1 = (cptr<T>)uniform_ptr(c.M[1]);
-      cptr<T> M2 = (cptr<T>)uniform_ptr(c.M[2]);
-      asm volatile("" : "+s"(M0), "+s"(M1), "+s"(M2));
-      slab_mode<T, N0, 1, P>(x, M0);
-      if constexpr (N1 > 1) slab_mode<T, N1, N0, P>(x, M1);
-      if constexpr (N2 > 1) slab_mode<T, N2, N0 * N1, P>(x, M2);
-    }
-    if constexpr (FUSED) {
-#pragma unroll
-      for (int J = 0; J < P; ++J) x[J] *= c.Vs[(base + c.AV * J) >> c.lgnv];
-      cptr<T> Q0 = (cptr<T>)uniform_ptr(c.Mt[0]);
-      cptr<T> Q1 = (cptr<T>)uniform_ptr(c.Mt[1]);
-      cptr<T> Q2 = (cptr<T>)uniform_ptr(c.Mt[2]);
-      asm volatile("" : "+s"(Q0), "+s"(Q1), "+s"(Q2));
-      slab_mode<T, N0, 1, P>(x, Q0);
-      if constexpr (N1 > 1) slab_mode<T, N1, N0, P>(x, Q1);
-      if constexpr (N2 > 1) slab_mode<T, N2, N0 * N1, P>(x, Q2);
-    }
-    if (c.scatter) {
-#pragma unroll
-      for (int J = 0; J < P; ++J) {
-        const int m = c.Ms[P * b + J];
-        if (m >= 0) atomicAdd(&c.hpsi[(int64_t)m * c.nvec + c.v0 + av], c.w * (double)x[J]);
-      }
-    } else {
-#pragma unroll
-      for (int J = 0; J < P; ++J) c.X[base + c.AV * J] = x[J];
-    }
-  }
-}
-
-template <class T, bool FUSED>
-__device__ __forceinline__ void dispatch_split(const SCtx<T>& c, int shape) {
-  switch (shape) {
-#define SG4_SCASE(id, a, b, d) case id: split_pass<T, a, b, d, FUSED>(c); return;
-    SG4_SHAPES(SG4_SCASE)
-#undef SG4_SCASE
-    default: return;
-  }
-}
-
-// Potential part of one term for one chunk of vectors (every active mode
-// square and grouped): stage map and V -> basis->grid groups 0..G-2 (group 0
-// gathering from the packed block) -> fused group G-1 -> grid->basis groups
-// G-2..0 (group 0 scatter-adding into the result).  A term with one group
-// does all of it in one register pass.
-template <class T>
-__device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km, const Item& it,
-                                              T* lds) {
-  const TermDev* Td = &a.terms[it.term];
-  const int lg = it.lgnv, nv = 1 << lg;
-  const int32_t* map = a.map + Td->boff;
-  const T* V = km.V + Td->qoff;
-  const double w = Td->wgt;
-  if (Td->nact == 0) {  // the one-point term
-    if ((int)threadIdx.x < nv) {
-      const int m = map[0];
-      if (m >= 0) {
-        const int64_t r = (int64_t)m * a.nvec + it.v0 + threadIdx.x;
-        atomicAdd(&a.hpsi[r], w * (double)(V[0] * (T)a.psi[r]));
-      }
-    }
-    return;
-  }
-  const int N = Td->nq;
-  const int NE = N * nv;
-  const int nt = blockDim.x;
-  T* X = lds;
-  T* Vs = lds + NE;
-  int32_t* Ms = (int32_t*)(Vs + N);
-  for (int p0 = threadIdx.x; p0 < N; p0 += GK * nt) {
-    int m[GK];
-    T v[GK];
-#pragma unroll
-    for (int k = 0; k < GK; ++k) {
-      const int p = p0 + k * nt;
-      m[k] = (p < N) ? map[p] : -1;
-      v[k] = (p < N) ? V[p] : (T)0;
-    }
-#pragma unroll
-    for (int k = 0; k < GK; ++k) {
-      const int p = p0 + k * nt;
-      if (p < N) {
-        Ms[p] = m[k];
-        Vs[p] = v[k];
-      }
-    }
-  }
-  __syncthreads();
-  const int G = Td->ngrp;
-  SCtx<T> c;
-  c.X = X;
-  c.Vs = Vs;
-  c.Ms = Ms;
-  c.psi = a.psi;
-  c.hpsi = a.hpsi;
-  c.w = w;
-  c.nvec = a.nvec;
-  c.v0 = it.v0;
-  c.lgnv = lg;
-  int Ag[MAXACT];
-  {
-    int A = 1;
-    for (int g = 0; g < G; ++g) {
-      Ag[g] = A;
-      const int p1 = (g + 1 < G) ? Td->gfirst[g + 1] : Td->nact;
-      for (int pm = Td->gfirst[g]; pm < p1; ++pm) A *= a.md[Td->mat[pm]].nq;
-    }
-  }
-  auto setup = [&](int g, bool fused, bool btw) {
-    const int p0 = Td->gfirst[g];
-    const int p1 = (g + 1 < G) ? Td->gfirst[g + 1] : Td->nact;
-    int P = 1;
-    for (int q = 0; q < 3; ++q) {
-      const int pm = p0 + q;
-      if (pm < p1) {
-        const MatDev* md = &a.md[Td->mat[pm]];
-        P *= md->nq;
-        c.M[q] = km.mats + (btw ? md->BtW : md->B);
-        c.Mt[q] = km.mats + md->BtW;
-      } else {
-        c.M[q] = c.M[0];
-        c.Mt[q] = c.Mt[0];
-      }
-    }
-    (void)fused;
-    c.AV = Ag[g] * nv;
-    c.invAV = 1.0f / (float)c.AV;
-    c.nfib = NE / P;
-  };
-  // group 0 gathers; the last grid->basis pass (group 0) scatter-adds
-  for (int g = 0; g + 1 < G; ++g) {
-    setup(g, false, false);
-    c.gather = (g == 0);
-    c.scatter = 0;
-    dispatch_split<T, false>(c, Td->gshape[g]);
-    __syncthreads();
-  }
-  setup(G - 1, true, false);
-  c.gather = (G == 1);
-  c.scatter = (G == 1);
-  dispatch_split<T, true>(c, Td->gshape[G - 1]);
-  for (int g = G - 2; g >= 0; --g) {
-    __syncthreads();
-    setup(g, false, true);
-    c.gather = 0;
-    c.scatter = (g == 0);
-    dispatch_split<T, false>(c, Td->gshape[g]);
-  }
-}
-
-template <class T>
-__device__ __forceinline__ void split_items(const KArgs& a, const KMats<T>& km) {
-  extern __shared__ double lds_d[];
-  T* lds = (T*)lds_d;
-  for (int w = blockIdx.x; w < a.nitems; w += gridDim.x) {
-    if (w != (int)blockIdx.x) __syncthreads();   // LDS reuse between items
-    const Item it = a.items[w];
-    if (it.term < 0) continue;
-    term_t1_split<T>(a, km, it, lds);
-  }
-}
-// the main item list, and (separate names for profiling) the terms that need
-// more LDS than the budget.  The FP32 kernels fit 168 VGPRs and may run 768
-// threads (three waves per SIMD at one workgroup per CU).
-constexpr int NT32 = 768;
-__global__ __launch_bounds__(NT) void k_hpsi_t1s_f64(KArgs a, KMats<double> km) { split_items(a, km); }
-__global__ __launch_bounds__(NT32) void k_hpsi_t1s_f32(KArgs a, KMats<float> km) { split_items(a, km); }
-__global__ __launch_bounds__(NT) void k_hpsi_t1s_big_f64(KArgs a, KMats<double> km) { split_items(a, km); }
-__global__ __launch_bounds__(NT32) void k_hpsi_t1s_big_f32(KArgs a, KMats<float> km) { split_items(a, km); }
-
-// Kinetic part of the split path: hpsi(r, v) = sum_e K(r, col_e) psi(col_e, v)
-// (CSR rows, the diagonal first); one thread per (row, vector), plain stores,
-// so it also initialises the result that the term kernel then adds to.
-__global__ void k_kin_rows(const int64_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                           const double* __restrict__ val, const double* __restrict__ psi,
-                           double* __restrict__ hpsi, int64_t nrows, int nvec) {
-  const int64_t n = nrows * nvec;
-  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n;
-       idx += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = idx / nvec;
-    const int v = (int)(idx - r * nvec);
-    const int64_t e1 = ptr[r + 1];
-    double s0 = 0.0, s1 = 0.0;
-    int64_t e = ptr[r];
-    for (; e + 1 < e1; e += 2) {
-      s0 = fma(val[e], psi[(int64_t)col[e] * nvec + v], s0);
-      s1 = fma(val[e + 1], psi[(int64_t)col[e + 1] * nvec + v], s1);
-    }
-    if (e < e1) s0 = fma(val[e], psi[(int64_t)col[e] * nvec + v], s0);
-    hpsi[idx] = s0 + s1;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Type-1 operator: one workgroup per work item.  T is the type of the mode
-// contractions (double: parity mode; float: the FP32 mode); gather and
-// scatter stay in FP64.
-// ---------------------------------------------------------------------------
-
-template <class T, bool BIG>
-__device__ __forceinline__ void term_t1(const KArgs& a, const KMats<T>& km, const Item& it,
-                                        T* lds) {
-  const TermDev* Td = &a.terms[it.term];
-  const int lg = it.lgnv, nv = 1 << lg;
-  const int nact = Td->nact;
-  const int32_t* map = a.map + Td->boff;
-  const T* V = km.V + Td->qoff;
-
-  if (nact == 0) {  // the one-point term
-    if ((int)threadIdx.x < nv) {
-      const int m = map[0];
-      if (m >= 0) {
-        const int64_t r = (int64_t)m * a.nvec + it.v0 + threadIdx.x;
-        atomicAdd(&a.hpsi[r], Td->wgt * (double)(((T)V[0] + (T)Td->vshift) * (T)a.psi[r]));
-      }
-    }
-    return;
-  }
-
-  T* buf0 = lds;
-  T* buf1 = BIG ? (T*)a.scratch + (int64_t)it.slot * a.slot_size : lds + (size_t)Td->nq * nv;
-
-  PassCtx<T> c;
-  c.map = map; c.psi = a.psi; c.hpsi = a.hpsi; c.nvec = a.nvec; c.v0 = it.v0;
-  c.wgt = Td->wgt; c.V = V; c.vshift = (T)Td->vshift; c.lgnv = lg; c.coef = 1;
-
-  // ---- basis -> grid ----
-  int A = 1;             // product of already transformed sizes
-  int Bafter = Td->nb;   // product of untransformed sizes (incl. current)
-  T* cur = nullptr;
-  T* nxt = buf0;
-  for (int p = 0; p < nact; ++p) {
-    const MatDev* md = &a.md[Td->mat[p]];
-    const int nq = md->nq, nb = md->nb;
-    Bafter /= nb;
-    c.MR = km.mats + md->B;
-    c.in = cur; c.out = nxt;
-    c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
-    c.nfib = A * nv * Bafter;
-    if (p == 0) dispatch_pass<T, !BIG, IN_GATHER, OUT_STORE>(c, nb, nq);
-    else dispatch_pass<T, !BIG, IN_LDS, OUT_STORE>(c, nb, nq);
-    __syncthreads();
-    A *= nq;
-    cur = nxt;
-    nxt = (nxt == buf0) ? buf1 : buf0;
-  }
-  // ---- grid: V psi + sum_k c_k d2_k psi ----
-  T* pg = cur;
-  T* acc = nxt;
-  {
-    int Ab = 1, Aa = Td->nq;
-    for (int p = 0; p < nact; ++p) {
-      const MatDev* md = &a.md[Td->mat[p]];
-      const int nq = md->nq;
-      Aa /= nq;
-      c.MR = km.mats + md->D2;
-      c.in = pg; c.out = acc;
-      c.A = Ab; c.AV = Ab * nv; c.invAV = 1.0f / (float)c.AV;
-      c.nfib = Ab * nv * Aa;
-      c.coef = km.ck[Td->kmode[p]];
-      if (p == 0) dispatch_pass<T, !BIG, IN_LDS, OUT_VSTORE>(c, nq, nq);
-      else dispatch_pass<T, !BIG, IN_LDS, OUT_ADD>(c, nq, nq);
-      __syncthreads();
-      Ab *= nq;
-    }
-  }
-  // ---- grid -> basis, last pass scatters ----
-  c.coef = 1;
-  cur = acc;
-  nxt = pg;
-  A = 1;
-  int Aq = Td->nq;
-  for (int p = 0; p < nact; ++p) {
-    const MatDev* md = &a.md[Td->mat[p]];
-    const int nq = md->nq, nb = md->nb;
-    Aq /= nq;
-    c.MR = km.mats + md->BtW;
-    c.in = cur; c.out = nxt;
-    c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
-    c.nfib = A * nv * Aq;
-    if (p == nact - 1) {
-      dispatch_pass<T, !BIG, IN_LDS, OUT_SCATTER>(c, nq, nb);
-    } else {
-      dispatch_pass<T, !BIG, IN_LDS, OUT_STORE>(c, nq, nb);
-      __syncthreads();
-    }
-    A *= nb;
-    T* t = cur; cur = nxt; nxt = t;
-  }
-}
-
-// The grid is either one workgroup per item or a persistent grid that walks
-// the (cost-sorted) item list with stride gridDim.x.
-__global__ __launch_bounds__(NT) void k_hpsi_t1_f64(KArgs a, KMats<double> km) {
-  extern __shared__ double lds[];
-  for (int w = blockIdx.x; w < a.nitems; w += gridDim.x) {
-    if (w != (int)blockIdx.x) __syncthreads();   // LDS reuse between items
-    const Item it = a.items[w];
-    if (it.term < 0) continue;
-    if (it.big) term_t1<double, true>(a, km, it, lds);
-    else if (a.terms[it.term].ngrp > 0) term_t1_grouped<double>(a, km, it, lds);
-    else term_t1<double, false>(a, km, it, lds);
-  }
-}
-
-__global__ __launch_bounds__(NT) void k_hpsi_t1_f32(KArgs a, KMats<float> km) {
-  extern __shared__ double lds_d[];
-  float* lds = (float*)lds_d;
-  for (int w = blockIdx.x; w < a.nitems; w += gridDim.x) {
-    if (w != (int)blockIdx.x) __syncthreads();
-    const Item it = a.items[w];
-    if (it.term < 0) continue;
-    if (it.big) term_t1<float, true>(a, km, it, lds);
-    else if (a.terms[it.term].ngrp > 0) term_t1_grouped<float>(a, km, it, lds);
-    else term_t1<float, false>(a, km, it, lds);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Type-10 operator (curvilinear kinetic d1 G d1), FP64
-// (sub_TabOpPsi_OF_ONEDP_FOR_SGtype4 case 10, SRC/sub_Operator/sub_OpPsi_SG4.f90
-// :1546-1649):
-//   u = sqRJ psi ; u_j = d1_j u ; w_i = Jac sum_j G(j,i) u_j ;
-//   H psi = -1/2 sum_i d1_i w_i / (Jac sqRJ) + V psi
-// Every mode takes part (a one-point mode has a 1x1 d1).  LDS holds D + 3
-// buffers of nq_t x nv: two for the transforms, the kinetic accumulator, and
-// one per derivative u_j.
-// ---------------------------------------------------------------------------
-struct KMats10 {
-  const double* mats;
-  const double* V;
-  const double* G;
-  const double* Jac;
-  const double* sqRJ;
-};
-
-__global__ __launch_bounds__(NT) void k_hpsi_t10_f64(KArgs a, KMats10 km) {
-  extern __shared__ double lds[];
-  const Item it = a.items[blockIdx.x];
-  const TermDev* Td = &a.terms[it.term];
-  const int lg = it.lgnv, nv = 1 << lg;
-  const int D = Td->nact;
-  const size_t SZ = (size_t)Td->nq * nv;
-  double* b0 = lds;
-  double* b1 = lds + SZ;
-  double* O = lds + 2 * SZ;
-  double* U = lds + 3 * SZ;
-  const int64_t q0 = Td->qoff;
-
-  PassCtx<double> c;
-  c.map = a.map + Td->boff; c.psi = a.psi; c.hpsi = a.hpsi; c.nvec = a.nvec; c.v0 = it.v0;
-  c.wgt = Td->wgt; c.V = km.V + q0; c.vshift = 0.0; c.lgnv = lg; c.coef = 1.0;
-
-  // ---- basis -> grid ----
-  int A = 1, Bafter = Td->nb;
-  double* cur = nullptr;
-  double* nxt = b0;
-  for (int p = 0; p < D; ++p) {
-    const MatDev* md = &a.md[Td->mat[p]];
-    const int nq = md->nq, nb = md->nb;
-    Bafter /= nb;
-    c.MR = km.mats + md->B;
-    c.in = cur; c.out = nxt;
-    c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
-    c.nfib = A * nv * Bafter;
-    if (p == 0) dispatch_pass<double, true, IN_GATHER, OUT_STORE>(c, nb, nq);
-    else dispatch_pass<double, true, IN_LDS, OUT_STORE>(c, nb, nq);
-    __syncthreads();
-    A *= nq;
-    cur = nxt;
-    nxt = (nxt == b0) ? b1 : b0;
-  }
-  double* pg = cur;   // psi on the grid
-  double* S = nxt;    // scratch: u, then w_i
-  for (size_t e = threadIdx.x; e < SZ; e += blockDim.x) S[e] = km.sqRJ[q0 + (e >> lg)] * pg[e];
-  __syncthreads();
-  // ---- u_j = d1_j u ----
-  {
-    int Ab = 1, Aa = Td->nq;
-    for (int j = 0; j < D; ++j) {
-      const MatDev* md = &a.md[Td->mat[j]];
-      const int nq = md->nq;
-      Aa /= nq;
-      c.MR = km.mats + md->D1;
-      c.in = S; c.out = U + (size_t)j * SZ;
-      c.A = Ab; c.AV = Ab * nv; c.invAV = 1.0f / (float)c.AV;
-      c.nfib = Ab * nv * Aa;
-      dispatch_pass<double, true, IN_LDS, OUT_STORE>(c, nq, nq);
-      Ab *= nq;
-    }
-  }
-  __syncthreads();
-  // ---- sum_i d1_i (Jac sum_j G(j,i) u_j) ----
-  {
-    int Ab = 1, Aa = Td->nq;
-    for (int i = 0; i < D; ++i) {
-      for (size_t e = threadIdx.x; e < SZ; e += blockDim.x) {
-        const int64_t q = q0 + (int64_t)(e >> lg);
-        const double* Gq = km.G + (int64_t)D * D * q;
-        double w = 0.0;
-        for (int j = 0; j < D; ++j) w = fma(Gq[j + D * i], U[(size_t)j * SZ + e], w);
-        S[e] = km.Jac[q] * w;
-      }
-      __syncthreads();
-      const MatDev* md = &a.md[Td->mat[i]];
-      const int nq = md->nq;
-      Aa /= nq;
-      c.MR = km.mats + md->D1;
-      c.in = S; c.out = O;
-      c.A = Ab; c.AV = Ab * nv; c.invAV = 1.0f / (float)c.AV;
-      c.nfib = Ab * nv * Aa;
-      if (i == 0) dispatch_pass<double, true, IN_LDS, OUT_STORE>(c, nq, nq);
-      else dispatch_pass<double, true, IN_LDS, OUT_ADD>(c, nq, nq);
-      __syncthreads();
-      Ab *= nq;
-    }
-  }
-  for (size_t e = threadIdx.x; e < SZ; e += blockDim.x) {
-    const int64_t q = q0 + (int64_t)(e >> lg);
-    O[e] = -0.5 * O[e] / (km.Jac[q] * km.sqRJ[q]) + km.V[q] * pg[e];
-  }
-  __syncthreads();
-  // ---- grid -> basis, last pass scatters ----
-  cur = O;
-  nxt = S;
-  A = 1;
-  int Aq = Td->nq;
-  for (int p = 0; p < D; ++p) {
-    const MatDev* md = &a.md[Td->mat[p]];
-    const int nq = md->nq, nb = md->nb;
-    Aq /= nq;
-    c.MR = km.mats + md->BtW;
-    c.in = cur; c.out = nxt;
-    c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
-    c.nfib = A * nv * Aq;
-    if (p == D - 1) {
-      dispatch_pass<double, true, IN_LDS, OUT_SCATTER>(c, nq, nb);
-    } else {
-      dispatch_pass<double, true, IN_LDS, OUT_STORE>(c, nq, nb);
-      __syncthreads();
-    }
-    A *= nb;
-    double* t = cur; cur = nxt; nxt = t;
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Grid-value gradient of <phi|H|psi>: both vectors transformed basis -> grid
-// (the first half of the H.psi pipeline), then the weighted product.
-// LDS: four buffers of nq_t x nv.
-// ---------------------------------------------------------------------------
-struct KGrad {
-  const double* mats;
-  const double* qw;        // 1-D quadrature weights
-  const int64_t* qw_off;   // (k*(Lmax+1)+l) -> offset
-  const double* phi;
-  double* grad;
-};
-
-__device__ void bg_transform(const KArgs& a, const double* mats, const TermDev* Td, const Item& it,
-                             const double* src, double* b0, double* b1, double** result) {
-  const int lg = it.lgnv, nv = 1 << lg;
-  PassCtx<double> c;
-  c.map = a.map + Td->boff; c.psi = src; c.hpsi = nullptr; c.nvec = a.nvec; c.v0 = it.v0;
-  c.wgt = 1.0; c.V = nullptr; c.vshift = 0.0; c.lgnv = lg; c.coef = 1.0;
-  int A = 1, Bafter = Td->nb;
-  double* cur = nullptr;
-  double* nxt = b0;
-  for (int p = 0; p < Td->nact; ++p) {
-    const MatDev* md = &a.md[Td->mat[p]];
-    const int nq = md->nq, nb = md->nb;
-    Bafter /= nb;
-    c.MR = mats + md->B;
-    c.in = cur; c.out = nxt;
-    c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
-    c.nfib = A * nv * Bafter;
-    if (p == 0) dispatch_pass<double, true, IN_GATHER, OUT_STORE>(c, nb, nq);
-    else dispatch_pass<double, true, IN_LDS, OUT_STORE>(c, nb, nq);
-    __syncthreads();
-    A *= nq;
-    cur = nxt;
-    nxt = (nxt == b0) ? b1 : b0;
-  }
-  *result = cur;
-}
-
-__global__ __launch_bounds__(NT) void k_grad_v(KArgs a, KGrad kg) {
-  extern __shared__ double lds[];
-  const Item it = a.items[blockIdx.x];
-  const TermDev* Td = &a.terms[it.term];
-  const int lg = it.lgnv, nv = 1 << lg;
-  const size_t SZ = (size_t)Td->nq * nv;
-  double* gphi = nullptr;
-  double* gpsi = nullptr;
-  if (Td->nact == 0) {   // one point, no transform
-    for (int v = threadIdx.x; v < nv; v += blockDim.x) {
-      const int m = a.map[Td->boff];
-      const int64_t r = (int64_t)m * a.nvec + it.v0 + v;
-      kg.grad[Td->qoff * a.nvec + it.v0 + v] = (m >= 0) ? Td->wgt * kg.phi[r] * a.psi[r] : 0.0;
-    }
-    return;
-  }
-  bg_transform(a, kg.mats, Td, it, kg.phi, lds, lds + SZ, &gphi);
-  bg_transform(a, kg.mats, Td, it, a.psi, lds + 2 * SZ, lds + 3 * SZ, &gpsi);
-  for (size_t e = threadIdx.x; e < SZ; e += blockDim.x) {
-    int pt = (int)(e >> lg);
-    double w = Td->wgt;
-    for (int p = 0; p < Td->nact; ++p) {
-      const MatDev* md = &a.md[Td->mat[p]];
-      const int i = pt % md->nq;
-      pt /= md->nq;
-      w *= kg.qw[kg.qw_off[Td->mat[p]] + i];
-    }
-    kg.grad[(Td->qoff + (int64_t)(e >> lg)) * a.nvec + it.v0 + (e & (nv - 1))] = w * gphi[e] * gpsi[e];
-  }
-}
 
 // V = V0 + eps * mu on every grid point (piecewise-constant field segment)
 __global__ void k_axpy_grid(const double* __restrict__ v0, const double* __restrict__ mu,
@@ -1849,12 +863,10 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
       if (P->timing && list == 0) HIP_TRY(hipEventRecord(P->ev0, P->stream));
       if (esize == 8) {
         KMats<double> km{P->d_mats.as<double>(), P->d_V.as<double>(), P->d_ck.as<double>()};
-        hipLaunchKernelGGL(list ? k_hpsi_t1s_big_f64 : k_hpsi_t1s_f64, dim3(gr), blk, lds,
-                           P->stream, b, km);
+        hipLaunchKernelGGL(k_hpsi_t1s_f64, dim3(gr), blk, lds, P->stream, b, km);
       } else {
         KMats<float> km{P->d_mats32.as<float>(), P->d_V32.as<float>(), P->d_ck32.as<float>()};
-        hipLaunchKernelGGL(list ? k_hpsi_t1s_big_f32 : k_hpsi_t1s_f32, dim3(gr), blk, lds,
-                           P->stream, b, km);
+        hipLaunchKernelGGL(k_hpsi_t1s_f32, dim3(gr), blk, lds, P->stream, b, km);
       }
       if (P->timing && list == 0) HIP_TRY(hipEventRecord(P->ev1, P->stream));
     }
@@ -1981,10 +993,6 @@ int sg4_plan_create_batch(int32_t ninst, const sg4_tables* tabs, int32_t max_nve
     (void)hipFuncSetAttribute((const void*)k_hpsi_t1s_f64,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
     (void)hipFuncSetAttribute((const void*)k_hpsi_t1s_f32,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
-    (void)hipFuncSetAttribute((const void*)k_hpsi_t1s_big_f64,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
-    (void)hipFuncSetAttribute((const void*)k_hpsi_t1s_big_f32,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
     (void)hipFuncSetAttribute((const void*)k_hpsi_t10_f64,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);

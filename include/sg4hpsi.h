@@ -39,7 +39,8 @@ extern "C" {
 /* precision modes of sg4_apply */
 #define SG4_PREC_FP64 0      /* FP64 throughout (parity mode, 1e-12 relative) */
 #define SG4_PREC_F32 1       /* mode contractions in FP32, FP64 gather/scatter */
-#define SG4_PREC_F32X2 2     /* split FP32 (hi+lo) contractions on MFMA, FP32 accumulation */
+#define SG4_PREC_F32X2 2     /* split FP32 (hi+lo) contractions on MFMA: reserved, applies
+                                 return SG4_ERR_UNSUPPORTED */
 
 typedef struct sg4_setup sg4_setup; /* host tables built by this library */
 typedef struct sg4_plan sg4_plan;   /* device state of one SG4 operator */
