@@ -271,3 +271,22 @@ def test_jacobi_device_grids_equal_host_grids(cuda):
     op_host = P.jacobi_triatomic_operator(6, max_nvec=4, device_grids=False)
     psi = np.random.default_rng(8).standard_normal((op_dev.basis.nb_ba, 3))
     assert rel(op_dev.apply_host(psi), op_host.apply_host(psi)) < 1e-14
+
+
+def test_launch_configurations_keep_parity(cuda):
+    """Every item order (plain, per term, classes of terms on one XCD) and the
+    768-thread FP32 kernel give the same operator; out-of-range launch
+    parameters are rejected."""
+    D, L, nvec = 6, 6, 16
+    b, op, o, V = hh_pair(D, L)
+    psi = np.random.default_rng(31).standard_normal((b.nb_ba, nvec))
+    ref = o.hpsi(psi, V, -0.5 * np.ones(D))
+    for xcd in (0, 1, 2, 3, 4):
+        op.set_config(nvec, E.PREC_FP64, 512, 163840, 1024, xcd)
+        assert rel(op.apply_host(psi), ref) < TOL, xcd
+    op.set_config(nvec, E.PREC_F32, 768, 163840, 1024, 2)
+    assert rel(op.apply_host(psi, E.PREC_F32), ref) < 1e-5
+    for bad in ((768, 65536, 0, 0, E.PREC_FP64), (512, 200000, 0, 0, E.PREC_FP64),
+                (512, 65536, -1, 0, E.PREC_FP64), (512, 65536, 0, 99, E.PREC_FP64)):
+        with pytest.raises(E.SG4Error):
+            op.set_config(nvec, bad[4], *bad[:4])
