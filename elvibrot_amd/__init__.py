@@ -62,8 +62,13 @@ class SG4Basis:
         _capi.check(lib().sg4_setup_tables(h, C.byref(self.tables)))
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().sg4_setup_destroy(self._h)
+        # at interpreter exit the module globals may already be gone; the
+        # process is ending then and the library frees nothing that matters
+        if getattr(self, "_h", None) and lib is not None:
+            try:
+                lib().sg4_setup_destroy(self._h)
+            except Exception:
+                pass
             self._h = None
 
     # numpy copies of the tables (host memory owned by the set-up)
@@ -164,8 +169,11 @@ class SG4Operator:
         _capi.check(lib().sg4_plan_set_operator(h, C.byref(op)))
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().sg4_plan_destroy(self._h)
+        if getattr(self, "_h", None) and lib is not None:
+            try:
+                lib().sg4_plan_destroy(self._h)
+            except Exception:
+                pass
             self._h = None
 
     @property
