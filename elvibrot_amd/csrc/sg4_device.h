@@ -14,6 +14,9 @@ constexpr int MAXACT = 16;
 constexpr int NT = 512;                 // maximum threads per workgroup (launch bound)
 constexpr int LDS_SMALL = 64 * 1024;    // LDS budget of the common class (2 WG/CU)
 constexpr int LDS_MAX = 160 * 1024;     // gfx950 LDS per workgroup
+// slab products along modes of >= this many points run row by row (one matrix
+// row live in scalar registers; whole 7x7 and larger matrices spilled them)
+constexpr int SLAB_ROW_OUTER = 7;
 
 struct MatDev {          // offsets (in doubles) of the 1-D matrices of (k,l)
   int64_t B, BtW, D1, D2;
@@ -245,6 +248,25 @@ struct GCtx {
 // y = M x along one mode of the slab (stride S, N entries), in place
 template <class T, int N, int S, int P>
 __device__ __forceinline__ void slab_mode(T (&x)[P], cptr<T> M) {
+  if constexpr (N >= SLAB_ROW_OUTER) {
+    // row by row: one row of M (2N scalar registers) is live at a time
+    // instead of the whole matrix, at the price of a second slab in VGPRs
+    T y[P];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+#pragma unroll
+      for (int o = 0; o < P / N; ++o) {
+        const int base = (o % S) + (o / S) * S * N;
+        T acc = 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) acc = fmat(M[i * N + j], x[base + S * j], acc);
+        y[base + S * i] = acc;
+      }
+    }
+#pragma unroll
+    for (int J = 0; J < P; ++J) x[J] = y[J];
+    return;
+  }
 #pragma unroll
   for (int o = 0; o < P / N; ++o) {
     const int base = (o % S) + (o / S) * S * N;
