@@ -32,6 +32,7 @@ import torch
 from scipy.special import jv
 
 from . import PREC_FP64, SG4Operator, SG4Batch
+from .psi_io import write_restart
 
 
 def cheby_coefficients(alpha, n):
@@ -173,13 +174,23 @@ def spectral_bounds(op, nsteps=60, seed=1, device="cuda:0", margin=0.05):
     return lo - margin * d, hi + margin * d
 
 
-def propagate_field(prop, psi0, eps, steps_per_segment, store_every=None, npolys=None):
+def propagate_field(prop, psi0, eps, steps_per_segment, store_every=None, npolys=None,
+                    restart_path=None, restart_every=1, t0=0.0):
     """Forward propagation through the segments; returns the final state, the
     checkpoints {segment index: state at its start} (if store_every is set)
-    and the number of polynomials used per step (or the given ones)."""
+    and the number of polynomials used per step (or the given ones).
+
+    With ``restart_path``, the states and the time ``t0 + steps * dt`` are
+    written as the reference's restart file (psi_io.write_restart) every
+    ``restart_every`` steps and after the last one, as its march loop does
+    after every step (SRC/sub_propagation/sub_module_propa_march.f90:226).
+    A run is resumed from psi_io.read_restart's states with the segments
+    not yet done and ``t0`` = the time read."""
     psi = psi0
     cps = {}
     used = []
+    nstep = 0
+    nall = len(eps) * steps_per_segment
     for s, e in enumerate(eps):
         if store_every and s % store_every == 0:
             cps[s] = psi
@@ -188,6 +199,9 @@ def propagate_field(prop, psi0, eps, steps_per_segment, store_every=None, npolys
         for j in range(steps_per_segment):
             psi, n = prop.step(psi, npoly=None if npolys is None else npolys[s][j])
             nps.append(n)
+            nstep += 1
+            if restart_path is not None and (nstep % restart_every == 0 or nstep == nall):
+                write_restart(restart_path, t0 + nstep * prop.dt, psi)
         used.append(nps)
     npolys = used
     prop.set_field(0.0)

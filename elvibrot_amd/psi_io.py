@@ -22,9 +22,17 @@ Formats (SRC/sub_WP/sub_module_psi_io.f90):
   ``'X'``, nb_psi, nb_tot, 1, ``'X'``, nb_tot index records, one record
   ``T, coefficients`` per vector.
 
+* propagation restart (SaveWP_restart / ReadWP_restart,
+  SRC/sub_propagation/sub_module_propagation.f90:379-495): ``T nb_WP nb_tot``,
+  then every packet's complex coefficients list-directed.
+
+Complex values are written as list-directed complex constants ``(re,im)``;
+the readers take the forms list-directed input allows (blanks or commas,
+``D`` exponents, repeat counts ``r*c``, values over several records).
 Only one electronic state and no extra index (``i_h = i_e = 1``) occur for
 the SG4 bases here.
 """
+import re
 import struct
 
 import numpy as np
@@ -42,6 +50,39 @@ def packed_indices(basis):
 
 def _fmt_real(x):
     return f"{x: .17E}"
+
+
+def _fmt_cplx(c):
+    # a complex constant of list-directed I/O: (re,im)
+    return f"({c.real:.17E},{c.imag:.17E})"
+
+
+_LD_TOKEN = re.compile(r"(?:(\d+)\*)?(\([^()]*\)|[^\s,()]+)")
+
+
+def _ld_values(text):
+    """The values of list-directed input: reals, complex constants ``(re,im)``
+    and repeat counts ``r*value``, separated by blanks, commas or line ends;
+    Fortran ``D`` exponents are accepted."""
+    out = []
+    for rpt, tok in _LD_TOKEN.findall(text):
+        if tok.startswith("("):
+            re_, im_ = tok[1:-1].split(",")
+            v = complex(float(re_.strip().replace("D", "E").replace("d", "e")),
+                        float(im_.strip().replace("D", "E").replace("d", "e")))
+        else:
+            v = float(tok.replace("D", "E").replace("d", "e"))
+        out.extend([v] * (int(rpt) if rpt else 1))
+    return out
+
+
+def _as_complex(vals, n):
+    """n complex numbers from list-directed values: complex constants, or
+    (re, im) pairs of reals as written by older files of this module."""
+    if len(vals) >= n and all(isinstance(v, complex) for v in vals[:n]):
+        return np.array(vals[:n], dtype=np.complex128), n
+    r = np.array(vals[:2 * n], dtype=np.float64)
+    return r[0::2] + 1j * r[1::2], 2 * n
 
 
 def write_psi(path, psi, basis, version=0, epsi=0.0, T=0.0, formatted=True):
@@ -93,9 +134,14 @@ def write_psi(path, psi, basis, version=0, epsi=0.0, T=0.0, formatted=True):
                 f.write(" " + " ".join(str(int(i)) for i in idx[r]) + " 1 1\n")
             for j in range(n):
                 v = psi[:, j]
-                vals = [T] + ([x for c in v for x in (c.real, c.imag)] if cplx else list(v))
-                for a in range(0, len(vals), 4):
-                    f.write(" " + " ".join(_fmt_real(x) for x in vals[a:a + 4]) + "\n")
+                if cplx:
+                    f.write(" " + _fmt_real(T) + "\n")
+                    for a in range(0, nb, 2):
+                        f.write(" " + " ".join(_fmt_cplx(c) for c in v[a:a + 2]) + "\n")
+                else:
+                    vals = [T] + list(v)
+                    for a in range(0, len(vals), 4):
+                        f.write(" " + " ".join(_fmt_real(x) for x in vals[a:a + 4]) + "\n")
         else:
             raise ValueError("version must be 0 or 1")
 
@@ -185,17 +231,74 @@ def read_psi(path, basis, cplx=False, formatted=True):
     lst = [tuple(int(x) for x in lines[pos + r].split()[:D]) for r in range(nb)]
     pos += nb
     target = np.array([rows.get(t, -1) for t in lst])
-    toks = " ".join(lines[pos:]).replace("D", "E").split()
-    vals = np.array([float(x) for x in toks])
-    per = 1 + (2 * nb if cplx else nb)
+    vals = _ld_values("\n".join(lines[pos:]))
     psi = np.zeros((nbr, n), dtype=dt)
     ok = target >= 0
+    q = 0
     for j in range(n):
-        blk = vals[j * per:(j + 1) * per]
-        info["T"].append(float(blk[0]))
-        v = blk[1:]
+        info["T"].append(float(vals[q].real))
+        q += 1
         if cplx:
-            v = v[0::2] + 1j * v[1::2]
+            v, used = _as_complex(vals[q:], nb)
+        else:
+            v, used = np.array([x.real for x in vals[q:q + nb]]), nb
+        q += used
         psi[target[ok], j] = v[ok]
         info["skipped"] += int((~ok).sum())
     return psi, info
+
+
+def write_restart(path, T, psis):
+    """Propagation restart file (SaveWP_restart,
+    SRC/sub_propagation/sub_module_propagation.f90:379-427; written after every
+    WPdeltaT by the march loop, sub_module_propa_march.f90:226): the record
+    ``T nb_WP nb_tot``, then every packet's complex coefficients (packed
+    basis order) list-directed, each packet starting a new record.
+    ``psis``: (nb_WP, nb_tot) complex, numpy or torch."""
+    if hasattr(psis, "detach"):
+        psis = psis.detach().cpu().numpy()
+    psis = np.asarray(psis, dtype=np.complex128)
+    if psis.ndim == 1:
+        psis = psis[None, :]
+    nwp, nb = psis.shape
+    with open(path, "w") as f:
+        f.write(f" {_fmt_real(T)} {nwp} {nb}\n")
+        for j in range(nwp):
+            v = psis[j]
+            for a in range(0, nb, 2):
+                f.write(" " + " ".join(_fmt_cplx(c) for c in v[a:a + 2]) + "\n")
+
+
+def read_restart(path, nb_wp, nb_tot):
+    """ReadWP_restart (sub_module_propagation.f90:429-495): returns (T, psis)
+    with psis (nb_wp, nb_tot) complex.  A file whose first record does not
+    hold ``T nb_WP nb_tot`` means no restart, as in the reference (a warning
+    there, T0 = 0): (0.0, None).  Sizes different from nb_wp, nb_tot raise, as
+    the reference stops."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    try:
+        head = _ld_values(lines[0]) if lines else []
+        T, nwp_f, nb_f = float(head[0].real), int(head[1].real), int(head[2].real)
+        if head[1].imag or head[2].imag or head[1].real != nwp_f or head[2].real != nb_f:
+            raise ValueError
+    except (ValueError, IndexError, AttributeError):
+        return 0.0, None
+    if nwp_f != nb_wp or nb_f != nb_tot:
+        raise ValueError(f"inconsistent WP size values: file {nwp_f} x {nb_f}, "
+                         f"expected {nb_wp} x {nb_tot}")
+    psis = np.zeros((nb_wp, nb_tot), dtype=np.complex128)
+    # every packet is one list-directed read: it starts on a new record and
+    # reads nb_tot complex constants across as many records as they take
+    pos = 1
+    for j in range(nb_wp):
+        got = []
+        while len(got) < nb_tot:
+            if pos >= len(lines):
+                raise ValueError(f"restart file ends inside packet {j + 1}")
+            got.extend(_ld_values(lines[pos]))
+            pos += 1
+        if not all(isinstance(v, complex) for v in got[:nb_tot]):
+            raise ValueError(f"packet {j + 1}: complex constants (re,im) expected")
+        psis[j] = got[:nb_tot]
+    return T, psis

@@ -88,3 +88,108 @@ def test_read_into_smaller_basis_skips_absent_functions(tmp_path):
     rows = {tuple(r): n for n, r in enumerate(psi_io.packed_indices(big))}
     sel = [rows[tuple(r)] for r in psi_io.packed_indices(small)]
     assert np.array_equal(got[:, 0], psi[sel, 0])
+
+
+def test_version1_complex_is_written_as_list_directed_complex_constants(tmp_path):
+    """Write_Psi_nDBasis version 1 writes ``T, CvecB(:)`` list-directed
+    (sub_module_psi_io.f90:1621): every coefficient is a constant (re,im)."""
+    b = E.SG4Basis(2, 2)
+    psi = (np.arange(b.nb_ba) + 1j * (np.arange(b.nb_ba) + 0.5))[:, None]
+    p = tmp_path / "v1c.dat"
+    psi_io.write_psi(str(p), psi, b, version=1, T=2.0)
+    body = p.read_text().splitlines()[5 + b.nb_ba:]
+    assert float(body[0]) == 2.0
+    toks = " ".join(body[1:]).split()
+    assert len(toks) == b.nb_ba and all(t.startswith("(") and t.endswith(")") for t in toks)
+    re_, im_ = toks[3][1:-1].split(",")
+    assert float(re_) == 3.0 and float(im_) == 3.5
+
+
+def test_version1_reader_accepts_fortran_list_directed_forms(tmp_path):
+    """Fortran list-directed records: blanks and commas, D exponents, repeat
+    counts r*c, values over several records; and the (re, im) real pairs of
+    files written before the complex constants."""
+    b = E.SG4Basis(2, 2)
+    idx = psi_io.packed_indices(b) + 1
+    head = " &HEADERFILE\n VERSION_FILE=1,\n NB_PSI=1,\n NB_TOT=%d,\n /\n" % b.nb_ba
+    lst = "".join(" %d %d 1 1\n" % tuple(r) for r in idx)
+    n = b.nb_ba
+    vals = ["(%d.0D0 , -1.0d-1)" % k for k in range(n - 2)] + ["2*(0.0,0.0)"]
+    text = head + lst + "   3.0000000000000000     ,\n" + "\n".join(
+        ", ".join(vals[a:a + 3]) for a in range(0, len(vals), 3)) + "\n"
+    p = tmp_path / "fortran_v1.dat"
+    p.write_text(text)
+    got, info = psi_io.read_psi(str(p), b, cplx=True)
+    want = np.array([k - 0.1j for k in range(n - 2)] + [0, 0])
+    assert info["T"] == [3.0] and np.array_equal(got[:, 0], want)
+    # pairs of reals
+    pairs = " ".join("%d.0 -0.1" % k for k in range(n - 2)) + " 0.0 0.0 0.0 0.0"
+    p.write_text(head + lst + " 3.0 " + pairs + "\n")
+    got, _ = psi_io.read_psi(str(p), b, cplx=True)
+    assert np.array_equal(got[:, 0], want)
+
+
+def test_restart_round_trip_and_fortran_record_layout(tmp_path):
+    """SaveWP_restart / ReadWP_restart (sub_module_propagation.f90:379-495):
+    record ``T nb_WP nb_tot``, then each packet's complex coefficients from a
+    new record on."""
+    rng = np.random.default_rng(5)
+    psis = rng.standard_normal((3, 7)) + 1j * rng.standard_normal((3, 7))
+    p = tmp_path / "file_WP_restart"
+    psi_io.write_restart(str(p), 1.25, psis)
+    lines = p.read_text().splitlines()
+    head = lines[0].split()
+    assert float(head[0]) == 1.25 and head[1:] == ["3", "7"]
+    assert len(lines) == 1 + 3 * 4        # 7 constants, two per record, per packet
+    T, got = psi_io.read_restart(str(p), 3, 7)
+    assert T == 1.25 and np.array_equal(got, psis)
+
+
+def test_restart_reader_fortran_text_missing_time_and_size_checks(tmp_path):
+    p = tmp_path / "r"
+    # gfortran-style list-directed output, a packet over two records, a repeat count
+    p.write_text("   0.50000000000000000                2           3\n"
+                 " (  1.0000000000000000     , -2.0000000000000000     ) (0.5D0,0.25D0)\n"
+                 " (0.0,1.0)\n"
+                 " 3*(  0.0000000000000000     ,  0.0000000000000000     )\n")
+    T, got = psi_io.read_restart(str(p), 2, 3)
+    assert T == 0.5
+    assert np.array_equal(got, np.array([[1 - 2j, 0.5 + 0.25j, 1j], [0, 0, 0]]))
+    with pytest.raises(ValueError, match="inconsistent"):
+        psi_io.read_restart(str(p), 2, 4)
+    p.write_text(" (1.0,0.0) (2.0,0.0)\n")    # no T record: no restart, T0 = 0
+    assert psi_io.read_restart(str(p), 1, 2) == (0.0, None)
+
+
+def test_propagation_writes_restart_every_step_and_resumes(tmp_path):
+    """propagate_field's restart file holds the state and time after the last
+    written step; resuming from it with the remaining segments gives the same
+    final state as the uninterrupted run (a host-side stand-in propagator:
+    the file logic does not touch the GPU)."""
+    import torch
+    from elvibrot_amd.propagation import propagate_field
+
+    class Stub:
+        dt = 0.1
+
+        def __init__(self):
+            self.e = 0.0
+
+        def set_field(self, e):
+            self.e = e
+
+        def step(self, psi, npoly=None):
+            return psi * complex(np.cos(1 + self.e), -np.sin(1 + self.e)) + self.e, 3
+
+    psi0 = torch.tensor(np.random.default_rng(6).standard_normal((2, 5)) + 0j)
+    eps = [0.1, -0.2, 0.3]
+    p = tmp_path / "file_WP_restart"
+    full, _, _ = propagate_field(Stub(), psi0, eps, 2, restart_path=str(p), restart_every=4)
+    T, last = psi_io.read_restart(str(p), 2, 5)
+    assert abs(T - 0.6) < 1e-15 and np.array_equal(last, full.numpy())
+    part, _, _ = propagate_field(Stub(), psi0, eps[:2], 2, restart_path=str(p))
+    T, mid = psi_io.read_restart(str(p), 2, 5)
+    assert abs(T - 0.4) < 1e-15
+    rest, _, _ = propagate_field(Stub(), torch.tensor(mid), eps[2:], 2, restart_path=str(p), t0=T)
+    assert np.array_equal(rest.numpy(), full.numpy())
+    assert abs(psi_io.read_restart(str(p), 2, 5)[0] - 0.6) < 1e-15
