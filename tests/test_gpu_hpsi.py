@@ -125,6 +125,40 @@ def test_10d_L8_block16_properties(cuda):
     assert nrm(H3 - HX[3:4]) / nrm(HX[3:4]) < 1e-12
 
 
+def test_12d_L8_largest_range_instance_properties(cuda):
+    """The largest instance of the D x L range (12D, L=8: 125 970 terms,
+    120.2 M grid points, nb_ba 4 673 345), where the oracle would take hours:
+    the split path is taken, H is symmetric and linear, a vector applied alone
+    equals its column of the block, and the FP32 mode agrees with FP64 to its
+    precision."""
+    import torch
+    D, L, nvec = 12, 8, 4
+    b = E.SG4Basis(D, L)
+    assert (b.nterms, b.nb_ba) == (125970, 4673345)
+    op = E.SG4Operator(b, max_nvec=nvec)
+    assert op.split_info()[0]
+    g = torch.Generator(device=cuda).manual_seed(12)
+    X = torch.randn(nvec, b.nb_ba, device=cuda, dtype=torch.float64, generator=g)
+    HX = torch.empty_like(X)
+    op.apply(X, HX)
+    nrm = lambda a: torch.linalg.norm(a).item()
+    S = X @ HX.T
+    assert (torch.max(torch.abs(S - S.T)) / torch.max(torch.abs(S))).item() < 1e-12
+    Y = (0.5 * X[0:1] + 4 * X[2:3]).contiguous()
+    HY = torch.empty_like(Y)
+    op.apply(Y, HY)
+    lin = 0.5 * HX[0:1] + 4 * HX[2:3]
+    assert nrm(HY - lin) / nrm(lin) < 1e-12
+    H1 = torch.empty_like(X[1:2])
+    op.apply(X[1:2].contiguous(), H1)
+    assert nrm(H1 - HX[1:2]) / nrm(HX[1:2]) < 1e-12
+    H32 = torch.empty_like(X)
+    op.apply(X, H32, E.PREC_F32)
+    r32 = nrm(H32 - HX) / nrm(HX)
+    print(f"12D L8: FP32 vs FP64 2-norm rel {r32:.2e}")
+    assert r32 < 1e-5
+
+
 def test_errors_are_reported(cuda):
     b, op, o, V = hh_pair(3, 2, max_nvec=2)
     with pytest.raises(E.SG4Error):
