@@ -99,9 +99,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(terms would be sharded across GPUs with one all-reduce of the packed "
-              "block per apply; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path "
+              "(terms would be sharded across GPUs by flops, with one all-reduce of "
+              "the packed block per apply)", file=sys.stderr)
         sys.exit(2)
 
     import torch

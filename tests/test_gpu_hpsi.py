@@ -99,7 +99,7 @@ def test_10d_L8_parity_two_vectors(cuda):
 
 
 def test_10d_L8_block16_properties(cuda):
-    """Full BASELINE size: symmetry <phi|H psi> = <H phi|psi>, linearity, and
+    """The 10D L=8 block-16 workload: symmetry <phi|H psi> = <H phi|psi>, linearity, and
     agreement of every vector's result with the one-vector apply."""
     import torch
     D, L, nvec = 10, 8, 16
