@@ -894,6 +894,19 @@ extern "C" {
 
 const char* sg4_last_error(void) { return g_last_error.c_str(); }
 
+// The kernels read psi while they write or scatter-add into hpsi: the two
+// nb_ba x nvec blocks must be distinct memory.
+static bool blocks_overlap(const sg4_plan* p, int32_t nvec, const double* psi, const double* hpsi) {
+  if (nvec < 1) return false;
+  const uintptr_t a = (uintptr_t)psi, b = (uintptr_t)hpsi;
+  const uintptr_t n = (uintptr_t)p->nb_ba * (uintptr_t)nvec * sizeof(double);
+  if (a < b + n && b < a + n) {
+    set_error("psi and hpsi overlap");
+    return true;
+  }
+  return false;
+}
+
 int sg4_setup_create(int32_t D, int32_t L, int32_t LB, const sg4_mode_desc* modes,
                      sg4_setup** out) {
   if (!modes || !out) { set_error("null argument"); return SG4_ERR_ARG; }
@@ -1257,12 +1270,14 @@ int sg4_plan_set_jacobi_triatomic(sg4_plan* p, const double* prm) {
 
 int sg4_apply_rows(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_t prec) {
   if (!p || !psi || !hpsi) { set_error("null argument"); return SG4_ERR_ARG; }
+  if (blocks_overlap(p, nvec, psi, hpsi)) return SG4_ERR_ARG;
   return apply_rows(p, nvec, psi, hpsi, prec);
 }
 
 int sg4_apply(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_t prec) {
   if (!p || !psi || !hpsi) { set_error("null argument"); return SG4_ERR_ARG; }
   if (nvec < 1 || nvec > p->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
+  if (blocks_overlap(p, nvec, psi, hpsi)) return SG4_ERR_ARG;
   const int64_t n = p->nb_ba * nvec;
   double* ri = p->d_rows_in.as<double>();
   double* ro = p->d_rows_out.as<double>();
@@ -1441,6 +1456,7 @@ int sg4_apply_graph(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, 
   if (!p || !psi || !hpsi) { set_error("null argument"); return SG4_ERR_ARG; }
   int rc;
   if ((rc = check_apply(p, nvec, prec))) return rc;
+  if (blocks_overlap(p, nvec, psi, hpsi)) return SG4_ERR_ARG;
   hipStream_t user = p->stream;
   if (!p->own_stream) HIP_TRY(hipStreamCreateWithFlags(&p->own_stream, hipStreamNonBlocking));
   sg4_plan::GraphEntry* g = nullptr;

@@ -153,7 +153,10 @@ int sg4_plan_set_jacobi_triatomic(sg4_plan *p, const double *prm);
 
 /* Block H.psi: sub_TabOpPsi_FOR_SGtype4(Psi, OpPsi, para_Op)
  * (SRC/sub_Operator/sub_OpPsi_SG4.f90:681-702).  psi and hpsi are DEVICE
- * pointers to nb_ba x nvec column-major blocks; hpsi is overwritten. */
+ * pointers to nb_ba x nvec column-major blocks; hpsi is overwritten.  The two
+ * blocks must not overlap (distinct objects, as Psi and OpPsi are in the
+ * reference): overlapping blocks are rejected with SG4_ERR_ARG.  The same holds
+ * for sg4_apply_rows and sg4_apply_graph. */
 int sg4_apply(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);
 /* Same with HOST pointers (copies in and out over PCIe; synchronous). */
 int sg4_apply_host(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);

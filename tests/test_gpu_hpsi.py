@@ -165,6 +165,17 @@ def test_errors_are_reported(cuda):
         op.apply_host(np.zeros((b.nb_ba, 3)))
     with pytest.raises(E.SG4Error):
         op.apply_host(np.zeros((b.nb_ba, 1)), prec=7)
+    # psi and hpsi must be distinct blocks (the kernels read one while adding
+    # into the other): in place, or shifted by one row, is rejected
+    import torch
+    x = torch.zeros(3, b.nb_ba, device=cuda, dtype=torch.float64)
+    with pytest.raises(E.SG4Error, match="overlap"):
+        op.apply(x[0:2], x[0:2])
+    with pytest.raises(E.SG4Error, match="overlap"):
+        op.apply_rows(x[0:2].reshape(b.nb_ba, 2), x[1:3].reshape(b.nb_ba, 2))
+    y = torch.empty_like(x[0:2])
+    op.apply(x[0:2], y)                   # adjacent, distinct blocks are fine
+    op.apply(x[1:2], x[0:1])
 
 
 @pytest.mark.parametrize("D,L", [(6, 6), (10, 5)])
