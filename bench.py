@@ -6,15 +6,25 @@ HO bases nq(l) = nb(l) = 1 + 2l, block of 16 seeded orthonormal vectors.
 A step is one block application H.psi (sub_TabOpPsi_FOR_SGtype4) with the
 input block already resident in HBM.
 
-Prints ONE JSON line with the metric, the roofline of the dominant kernel
-(the term kernel's main launch: algorithmic flops per launch over the launch
-time measured with HIP events on the plan's stream; HBM traffic per launch
-from the committed rocprofv3 counter summary when one exists for this
-workload) and the CPU baseline (the oracle, OpenMP over terms, on a bounded
-sample of the same workload).  For the 10D L=8 block-16 workload the launch
-parameters are the best ones the autotuner found on MI355X (pinned, so that
-runs and profiles use the same launch; --autotune re-tunes); other workloads
-are autotuned before the timed region (--no-autotune keeps the defaults).
+The reported value is the split-precision mode (SG4_PREC_F32X2: mode
+contractions on the matrix cores with three bf16 limbs per FP32 operand and
+FP32 accumulation; kinetic part, term sum and result in FP64); the plain FP64
+apply of the same operator is timed beside it with the same number of steps,
+and their ratio is reported.  The eigenvalue error condition of the metric is
+measured by block Davidson runs for the 20 lowest levels: FP64, split
+precision followed by an FP64 phase, and split precision alone, each
+compared with the FP64 levels.
+
+One JSON line: the metric, the roofline of the dominant kernel (the term
+kernel's main launch: algorithmic flops per launch over the launch time
+measured with HIP events on the plan's stream, against the peak of the unit
+the kernel computes on; HBM traffic per launch from the committed rocprofv3
+counter summary when one exists for this workload) and the CPU baseline (the
+oracle, OpenMP over terms, on a bounded sample of the same workload, with the
+host CPU model).  For the 10D L=8 block-16 workload the launch parameters are
+the best ones the autotuner found on MI355X (pinned, so that runs and
+profiles use the same launch; --autotune re-tunes); other workloads are
+autotuned before the timed region (--no-autotune keeps the defaults).
 """
 import argparse
 import json
@@ -28,13 +38,28 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp64": 78.6, "f32": 157.3, "f32x2": 157.3}  # TFLOP/s (MI355X dense)
+# peak of the unit each mode's term kernel computes on (TFLOP/s, MI355X dense)
+PEAK = {"fp64": 78.6, "f32": 157.3, "f32x2": 2500.0}
+UNIT = {"fp64": "FP64 VALU (= FP64 MFMA peak)", "f32": "FP32 VALU",
+        "f32x2": "bf16 MFMA (dense)"}
 # sg4_plan_autotune's best launches for 10D L=8 block 16 on MI355X:
 # threads, LDS budget, persistent grid, item order (3: classes of terms with
 # equal levels in modes 3..D-1 on one XCD; 2: modes 2..D-1)
-PINNED = {(10, 8, 16): {"fp64": (512, 163840, 2048, 3), "f32": (768, 163840, 2048, 2)}}
+PINNED = {(10, 8, 16): {"fp64": (512, 163840, 2048, 3), "f32": (768, 163840, 2048, 2),
+                        "f32x2": (512, 163840, 1024, 3)}}
 PREC = {"fp64": 0, "f32": 1, "f32x2": 2}
-DTYPE = {"fp64": "f64", "f32": "f32", "f32x2": "f32x2"}
+DTYPE = {"fp64": "f64", "f32": "f32", "f32x2": "bf16x3-split/f32-acc"}
+KERNEL = {"fp64": "k_hpsi_t1s_f64", "f32": "k_hpsi_t1s_f32", "f32x2": "k_hpsi_mfma"}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(D, L, nvec, stride, min_seconds=10.0):
@@ -55,7 +80,7 @@ def cpu_baseline(D, L, nvec, stride, min_seconds=10.0):
         dt += time.perf_counter() - t0
         reps += 1
     return {"value": reps * nvec * frac / dt, "unit": "H.psi/s", "cores": O.num_threads(),
-            "kind": "port",
+            "kind": "port", "cpu_model": cpu_model(),
             "sample": f"{reps} x every {stride}th Smolyak term ({frac:.3f} of the flops) of the "
                       f"same {D}D L={L} block-{nvec} FP64 apply, {dt:.1f} s in total, "
                       f"scaled by the flop share"}
@@ -74,19 +99,18 @@ def measured_traffic(kernel, D, L, nvec):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--prec", default="fp64", choices=list(PREC),
-                    help="precision of the reported value; the other mode is timed beside it")
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--prec", default="f32x2", choices=list(PREC),
+                    help="precision of the reported value; FP64 is timed beside it "
+                         "(the FP32 VALU mode beside FP64)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch parameters")
     ap.add_argument("--autotune", action="store_true",
                     help="autotune even where a pinned launch exists for the workload")
     ap.add_argument("--launch", default=None,
                     help="threads,lds_budget,grid,xcd for the reported precision (skips its autotune)")
-    ap.add_argument("--davidson", type=int, default=0, metavar="NLEV",
-                    help="only run the block Davidson for NLEV levels in FP64 and in FP32->FP64 "
-                         "phases at the same workload and report time, H.psi count and the "
-                         "largest relative eigenvalue difference")
+    ap.add_argument("--nlev", type=int, default=20,
+                    help="levels of the Davidson eigenvalue check (0: skip it)")
     ap.add_argument("--host-entry", action="store_true",
                     help="also time sg4_apply_host (host buffers, PCIe included)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -120,33 +144,11 @@ def main():
     D, L, nvec = args.D, args.L, args.nvec
     b = E.SG4Basis(D, L)
     op = E.SG4Operator(b, max_nvec=nvec)
-    if args.davidson:
-        from elvibrot_amd.davidson import sub_propagation_Davidson
-        out = {"workload": f"{D}D Henon-Heiles SG4 L={L} Davidson, {args.davidson} levels, block {nvec}"}
-        for name, p in (("fp64", PREC["fp64"]), ("f32_then_fp64", PREC["f32"])):
-            if not args.no_autotune:
-                op.autotune(nvec, p)
-                op.autotune(nvec, PREC["fp64"])
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            r = sub_propagation_Davidson(op, args.davidson, max_it=400, conv_resi=1e-6,
-                                         conv_ene=1e-10, prec=p)
-            torch.cuda.synchronize()
-            out[name] = {"seconds": time.perf_counter() - t0, "iterations": r.it,
-                         "n_hpsi": r.n_hpsi, "n_hpsi_f32": getattr(r, "n_hpsi_low", 0),
-                         "converged": bool(np.all(r.converged)), "E": [float(e) for e in r.E]}
-        e64 = np.array(out["fp64"]["E"])
-        e32 = np.array(out["f32_then_fp64"]["E"])
-        out["max_rel_eigen_diff"] = float(np.max(np.abs(e32 - e64) / np.abs(e64)))
-        out["speedup_time_to_solution"] = out["fp64"]["seconds"] / out["f32_then_fp64"]["seconds"]
-        print(json.dumps(out), flush=True)
-        return
     g = torch.Generator(device="cuda:0").manual_seed(2026)
     X = torch.randn(b.nb_ba, nvec, device="cuda:0", dtype=torch.float64, generator=g)
     X, _ = torch.linalg.qr(X)                       # orthonormal block
     Xc = X.T.contiguous()                            # (nvec, nb_ba) = column-major block
     Y = torch.empty_like(Xc)
-    prec = PREC[args.prec]
 
     def choose_launch(pname):
         pinned = PINNED.get((D, L, nvec), {}).get(pname)
@@ -156,9 +158,6 @@ def main():
             op.set_config(nvec, PREC[pname], *pinned)
         elif not args.no_autotune:
             op.autotune(nvec, PREC[pname])
-
-    choose_launch(args.prec)
-    split = op.split_info()[0]
 
     def timed(p, steps, warmup):
         for _ in range(warmup):
@@ -175,24 +174,55 @@ def main():
         op.timing(False)
         return dt, float(np.mean(kt)) * 1e-3
 
+    # the reported mode
+    choose_launch(args.prec)
+    split = op.split_info()[0]
+    prec = PREC[args.prec]
     dt, kavg = timed(prec, args.steps, args.warmup)
     st = op.stats(nvec)
-    achieved = st["flops"] / kavg / 1e12
+    mf, ngm = op.mfma_stats() if args.prec == "f32x2" else (0.0, -1)
+    Yrep = Y.clone()
+    # the other precision, same number of steps
     other = None
+    oname = "fp64" if args.prec != "fp64" else "f32"
     if not args.no_other:
-        op_name = "fp64" if args.prec != "fp64" else "f32"
-        choose_launch(op_name)
-        dto, kavgo = timed(PREC[op_name], max(3, args.steps // 2), 2)
-        other = {"precision": op_name, "H·psi/s": nvec * max(3, args.steps // 2) / dto,
-                 "kernel_ms": kavgo * 1e3}
-    kname = ("k_hpsi_t1s_" if split else "k_hpsi_t1_") + ("f64" if args.prec == "fp64" else "f32")
-    traffic = measured_traffic(kname, D, L, nvec)
+        choose_launch(oname)
+        dto, kavgo = timed(PREC[oname], args.steps, args.warmup)
+        sto = op.stats(nvec)
+        other = {"precision": oname, "H·psi/s": nvec * args.steps / dto,
+                 "ms_per_step": dto / args.steps * 1e3, "kernel": KERNEL[oname],
+                 "kernel_ms": kavgo * 1e3,
+                 "roofline": {"achieved": sto["flops"] / kavgo / 1e12, "peak": PEAK[oname],
+                              "frac": sto["flops"] / kavgo / 1e12 / PEAK[oname],
+                              "unit": "TFLOP/s"}}
+        if oname == "fp64":   # H.psi error of the reported mode against FP64, same input
+            e2 = (torch.linalg.norm(Yrep - Y, dim=1) / torch.linalg.norm(Y, dim=1)).max().item()
+            other["hpsi_rel_err_2norm_of_reported_vs_fp64"] = e2
     # flops of the reference's algorithm (B->G, kinetic on the grid, G->B):
     # F1 = 3 sum_t S_t per vector, S_t = sum_k 2 N_t n_k
     lev = b.levels()
     ntab = {(k, l): b.basis1d(k, l)["nq"] for k in range(D) for l in range(L + 1)}
     nq = np.array([[ntab[k, int(l)] for k, l in enumerate(row)] for row in lev], dtype=np.float64)
     F1 = 3.0 * float(np.sum(2.0 * np.prod(nq, axis=1) * nq.sum(axis=1))) * nvec
+    kname = KERNEL[args.prec] if split or args.prec == "f32x2" else (
+        "k_hpsi_t1_" + ("f64" if args.prec == "fp64" else "f32"))
+    traffic = measured_traffic(kname, D, L, nvec)
+    achieved = st["flops"] / kavg / 1e12
+    roof = {"bound": "mfma" if args.prec == "f32x2" else "valu",
+            "achieved": achieved, "peak": PEAK[args.prec], "unit": "TFLOP/s",
+            "frac": achieved / PEAK[args.prec], "traffic": traffic,
+            "kernel": kname, "compute_unit": UNIT[args.prec], "split_path": split,
+            "kernel_ms": kavg * 1e3,
+            "algorithmic_flops_per_launch": st["flops"],
+            "algorithmic_bytes_per_launch": st["bytes"],
+            "hbm_GBps_algorithmic": st["bytes"] / kavg / 1e9,
+            "F1_reference_flops_per_apply": F1,
+            "F1_over_kernel_TFLOPs": F1 / kavg / 1e12}
+    if args.prec == "f32x2":
+        roof["mfma_executed_flops_per_launch"] = mf
+        roof["mfma_executed_TFLOPs"] = mf / kavg / 1e12
+        roof["mfma_executed_frac_of_peak"] = mf / kavg / 1e12 / PEAK["f32x2"]
+        roof["group_matrices"] = ngm
     line = {
         "metric": "H·psi/s 10D Smolyak L=8 block-16: split-FP32 MFMA vs FP64, @1e-8 rel eigen err",
         "value": nvec * args.steps / dt,
@@ -210,18 +240,39 @@ def main():
                    "D": D, "L": L, "nvec": nvec, "nterms": b.nterms, "nb_ba": b.nb_ba,
                    "precision": args.prec, "items": st["items"],
                    "launch": op.config(nvec, prec)},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK[args.prec],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK[args.prec], "traffic": traffic,
-                     "kernel": kname, "split_path": split,
-                     "kernel_ms": kavg * 1e3,
-                     "flops_per_launch": st["flops"], "algorithmic_bytes": st["bytes"],
-                     "hbm_GBps_algorithmic": st["bytes"] / kavg / 1e9},
+        "roofline": roof,
         "reference_algorithm": {"flops_per_apply": F1,
                                 "effective_TFLOPs": F1 / (dt / args.steps) / 1e12},
     }
     if other is not None:
         line["other_precision"] = other
-        line["speedup_vs_other"] = (nvec * args.steps / dt) / other["H·psi/s"]
+        line[f"speedup_vs_{oname}"] = (nvec * args.steps / dt) / other["H·psi/s"]
+    if args.nlev and args.prec != "fp64":
+        from elvibrot_amd.davidson import sub_propagation_Davidson
+        runs = {}
+        for name, p, refine in (("fp64", PREC["fp64"], True),
+                                (f"{args.prec}_then_fp64", prec, True),
+                                (f"{args.prec}_only", prec, False)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = sub_propagation_Davidson(op, args.nlev, max_it=400, conv_resi=1e-6,
+                                         conv_ene=1e-10, prec=p, refine=refine)
+            torch.cuda.synchronize()
+            runs[name] = {"seconds": time.perf_counter() - t0, "iterations": r.it,
+                          "n_hpsi": r.n_hpsi, "n_hpsi_reduced": getattr(r, "n_hpsi_low", 0),
+                          "E": np.array(r.E)}
+        e64 = runs["fp64"]["E"]
+        eig = {"levels": args.nlev, "E_fp64": [float(e) for e in e64]}
+        for name in runs:
+            if name == "fp64":
+                eig["fp64"] = {k: v for k, v in runs[name].items() if k != "E"}
+                continue
+            d = {k: v for k, v in runs[name].items() if k != "E"}
+            d["max_rel_eigen_err_vs_fp64"] = float(np.max(np.abs(runs[name]["E"] - e64) / np.abs(e64)))
+            eig[name] = d
+        eig["time_to_solution_speedup"] = (runs["fp64"]["seconds"] /
+                                           runs[f"{args.prec}_then_fp64"]["seconds"])
+        line["eigen_check"] = eig
     if args.host_entry:
         hp = np.ascontiguousarray(X.cpu().numpy())
         op.apply_host(hp, prec)

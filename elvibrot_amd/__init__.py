@@ -263,6 +263,13 @@ class SG4Operator:
         _capi.check(lib().sg4_plan_kernel_times(self._h, buf, max_n, C.byref(n)))
         return np.array(buf[: n.value], dtype=np.float64)
 
+    def mfma_stats(self):
+        """(flops the matrix cores executed in the last split-precision apply's
+        term launch, number of group matrices; -1 before the first such apply)"""
+        f, n = C.c_double(), C.c_int64()
+        _capi.check(lib().sg4_plan_mfma_stats(self._h, C.byref(f), C.byref(n)))
+        return f.value, n.value
+
     def stats(self, nvec):
         f, b, n = C.c_double(), C.c_double(), C.c_int64()
         _capi.check(lib().sg4_plan_stats(self._h, nvec, C.byref(f), C.byref(b), C.byref(n)))

@@ -20,7 +20,8 @@ EXPORTS = [
     "sg4_plan_set_stream", "sg4_plan_set_operator", "sg4_plan_set_henon_heiles", "sg4_plan_set_jacobi_triatomic", "sg4_apply",
     "sg4_apply_host", "sg4_apply_rows", "sg4_plan_stats", "sg4_plan_timing",
     "sg4_plan_kernel_times", "sg4_grad_V", "sg4_plan_set_field", "sg4_plan_autotune", "sg4_plan_config", "sg4_plan_set_config",
-    "sg4_apply_graph", "sg4_plan_set_option", "sg4_plan_split_info", "sg4_plan_destroy",
+    "sg4_apply_graph", "sg4_plan_set_option", "sg4_plan_split_info", "sg4_plan_mfma_stats",
+    "sg4_plan_destroy",
 ]
 
 
@@ -85,6 +86,7 @@ def lib():
         L.sg4_plan_set_jacobi_triatomic.argtypes = [vp, C.POINTER(C.c_double)]
         L.sg4_setup_packed.argtypes = [vp, C.POINTER(i32)]
         L.sg4_plan_split_info.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_int64)]
+        L.sg4_plan_mfma_stats.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.sg4_plan_destroy.argtypes = [vp]
         _lib = L
     return _lib

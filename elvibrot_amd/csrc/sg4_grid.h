@@ -22,13 +22,12 @@ __device__ __forceinline__ void term_t1(const KArgs& a, const KMats<T>& km, cons
   const T* V = km.V + Td->qoff;
 
   if (nact == 0) {  // the one-point term
-    if ((int)threadIdx.x < nv) {
-      const int m = map[0];
-      if (m >= 0) {
-        const int64_t r = (int64_t)m * a.nvec + it.v0 + threadIdx.x;
+    const int m = map[0];
+    if (m >= 0)
+      for (int v = threadIdx.x; v < nv; v += blockDim.x) {
+        const int64_t r = (int64_t)m * a.nvec + it.v0 + v;
         atomicAdd(&a.hpsi[r], Td->wgt * (double)(((T)V[0] + (T)Td->vshift) * (T)a.psi[r]));
       }
-    }
     return;
   }
 

@@ -32,7 +32,7 @@
 #include <string>
 #include <vector>
 
-#include "sg4_device.h"
+#include "sg4_mfma.h"
 
 namespace sg4 {
 
@@ -224,6 +224,8 @@ struct ItemPlan {
   int64_t nitems2 = 0;
   int lds2 = 0;
   double flops_main = 0;   // algorithmic flops of the main H.psi launch
+  double flops_mfma = 0;   // MFMA path: flops the matrix cores execute in that launch
+                           // (padded Kronecker tiles, six limb products)
 };
 
 // Launch configuration of the H.psi kernel; chosen by sg4_plan_autotune.
@@ -273,6 +275,7 @@ struct sg4_plan {
   double flops_per_vec = 0, bytes_fixed = 0;
   int64_t last_items = -1;   // work items of the last apply
   double last_flops = 0;     // algorithmic flops of the last apply's main H.psi launch
+  double last_flops_mfma = 0;  // MFMA path: flops the matrix cores executed in that launch
   bool grouping = true;      // register-blocked group passes for square terms
   // split type-1 path: kinetic sparse matrix K (CSR over packed rows) and the
   // host copies of the map and matrices it is assembled from
@@ -281,6 +284,12 @@ struct sg4_plan {
   std::vector<int32_t> h_map;
   std::vector<double> h_mats;
   sg4::DevBuf d_kptr, d_kcol, d_kval;
+  // split-precision MFMA path (SG4_PREC_F32X2): terms cut into mode groups,
+  // group matrices in A-operand lane order
+  bool mfma_valid = false;
+  std::vector<sg4::TermM> h_termm;
+  sg4::DevBuf d_termm, d_mtab;
+  int64_t nmtab = 0;
   // optional kernel timing: HIP events around the H.psi launch of each apply
   bool timing = false;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -294,6 +303,26 @@ struct sg4_plan {
 };
 
 namespace sg4 {
+
+// Captured graphs hold kernel arguments that point at plan buffers (item
+// lists, the term and matrix tables, the kinetic matrix, the FP32 copies) and
+// the launch configuration of their capture: every call that reallocates
+// those buffers or changes the configuration destroys them, and the next
+// sg4_apply_graph captures again.
+static void drop_graphs(sg4_plan* P) {
+  if (P->graphs.empty()) return;
+  if (P->own_stream) (void)hipStreamSynchronize(P->own_stream);
+  for (auto& g : P->graphs) (void)hipGraphExecDestroy(g.exec);
+  P->graphs.clear();
+}
+
+// The operator or the term set changed: everything derived from them.
+static void invalidate(sg4_plan* P) {
+  drop_graphs(P);
+  P->items.clear();
+  P->f32_valid = false;
+  P->mfma_valid = false;
+}
 
 // Greedy grouping of consecutive active modes into the shapes of
 // sg4_shapes.h (triples, then pairs, then singles); 0 if some mode is not
@@ -515,6 +544,7 @@ static int build_kinetic(sg4_plan* P) {
 }
 
 static int build_terms(sg4_plan* P) {
+  P->mfma_valid = false;
   P->h_terms.resize(P->nterms);
   std::vector<double> ck = P->h_ck;
   if ((int)ck.size() < P->nck) ck.assign(P->nck, 0.0);
@@ -561,6 +591,79 @@ static int build_terms(sg4_plan* P) {
   return build_kinetic(P);
 }
 
+struct HItem { Item it; double cost; bool large; };
+
+// Item order of a launch: by decreasing cost, or (xcd >= 1) dealt to the
+// eight XCDs (workgroups go round-robin to the XCDs) so that the items of a
+// term (xcd = 1) or of a class of terms with equal levels in modes
+// xcd..D-1 (xcd >= 2) run on one XCD and share its L2 for their gathers.
+static std::vector<Item> order_items(const sg4_plan* P, const std::vector<HItem>& all, bool large,
+                                     int xcd) {
+  std::vector<Item> h;
+  if (!xcd) {
+    for (auto& x : all)
+      if (x.large == large) h.push_back(x.it);
+    return h;
+  }
+  std::vector<std::vector<Item>> lane(8);
+  double acc[8] = {0};
+  if (xcd >= 2) {
+    // terms with equal levels in modes xcd..D-1 share the rows that agree in
+    // those modes: put such a class on one XCD, its items next to each other
+    // (classes by decreasing largest item, items by decreasing cost)
+    std::map<std::vector<int32_t>, int> cls_of;
+    std::vector<std::vector<const HItem*>> cls;
+    for (auto& x : all) {
+      if (x.large != large) continue;
+      int64_t t = x.it.term;
+      size_t ii = 0;
+      while (ii + 1 < P->inst.size() && t >= P->inst[ii + 1].term0) ++ii;
+      const sg4_plan::Inst& I = P->inst[ii];
+      std::vector<int32_t> key(1, (int32_t)ii);
+      for (int k = xcd; k < I.D; ++k) key.push_back(P->h_levels[I.lev0 + (t - I.term0) * I.D + k]);
+      auto f = cls_of.find(key);
+      int c;
+      if (f == cls_of.end()) {
+        c = (int)cls.size();
+        cls_of[key] = c;
+        cls.emplace_back();
+      } else {
+        c = f->second;
+      }
+      cls[c].push_back(&x);
+    }
+    for (auto& members : cls) {
+      double w = 0;
+      for (const HItem* x : members) w += x->cost;
+      const int l = (int)(std::min_element(acc, acc + 8) - acc);
+      for (const HItem* x : members) lane[l].push_back(x->it);
+      acc[l] += w;
+    }
+  } else {
+    std::map<int32_t, int> owner;
+    for (auto& x : all) {
+      if (x.large != large) continue;
+      auto o = owner.find(x.it.term);
+      int l;
+      if (o == owner.end()) {
+        l = (int)(std::min_element(acc, acc + 8) - acc);
+        owner[x.it.term] = l;
+      } else {
+        l = o->second;
+      }
+      lane[l].push_back(x.it);
+      acc[l] += x.cost;
+    }
+  }
+  size_t m = 0;
+  for (auto& v : lane) m = std::max(m, v.size());
+  Item pad{};
+  pad.term = -1;
+  for (size_t i = 0; i < m; ++i)
+    for (int l = 0; l < 8; ++l) h.push_back(i < lane[l].size() ? lane[l][i] : pad);
+  return h;
+}
+
 // Work items for a block of nvec vectors: for every term the vectors are cut
 // into power-of-two chunks so that the work buffers fit the LDS budget; a
 // term too large for that even with one vector keeps its second buffer in a
@@ -581,8 +684,7 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
       (split ? 1 : 0);
   auto itp = P->items.find(key);
   if (itp != P->items.end()) { *out = itp->second.get(); return SG4_OK; }
-  struct HI { Item it; double cost; bool large; };
-  std::vector<HI> all;
+  std::vector<HItem> all;
   double flops_main = 0;
   all.reserve(P->nterms * 2);
   int64_t nbig = 0, slot = 0;
@@ -602,7 +704,7 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
       for (int v0 = 0; v0 < nvec;) {
         int g = gmax;
         while ((1 << g) > nvec - v0) --g;
-        HI h{};
+        HItem h{};
         h.it.term = (int32_t)t;
         h.it.v0 = (int16_t)v0;
         h.it.lgnv = (int8_t)g;
@@ -644,7 +746,7 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
     while (v0 < nvec) {
       int g = gmax;
       while ((1 << g) > nvec - v0) --g;
-      HI h{};
+      HItem h{};
       h.it.term = (int32_t)t;
       h.it.v0 = (int16_t)v0;
       h.it.lgnv = (int8_t)g;
@@ -657,74 +759,9 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
       v0 += 1 << g;
     }
   }
-  std::stable_sort(all.begin(), all.end(), [](const HI& x, const HI& y) { return x.cost > y.cost; });
-  auto order = [&](bool large) {
-    std::vector<Item> h;
-    if (!xcd) {
-      for (auto& x : all)
-        if (x.large == large) h.push_back(x.it);
-      return h;
-    }
-    std::vector<std::vector<Item>> lane(8);
-    double acc[8] = {0};
-    if (xcd >= 2) {
-      // terms with equal levels in modes xcd..D-1 share the rows that agree in
-      // those modes: put such a class on one XCD, its items next to each other
-      // (classes by decreasing largest item, items by decreasing cost)
-      std::map<std::vector<int32_t>, int> cls_of;
-      std::vector<std::vector<const HI*>> cls;
-      for (auto& x : all) {
-        if (x.large != large) continue;
-        int64_t t = x.it.term;
-        size_t ii = 0;
-        while (ii + 1 < P->inst.size() && t >= P->inst[ii + 1].term0) ++ii;
-        const sg4_plan::Inst& I = P->inst[ii];
-        std::vector<int32_t> key(1, (int32_t)ii);
-        for (int k = xcd; k < I.D; ++k) key.push_back(P->h_levels[I.lev0 + (t - I.term0) * I.D + k]);
-        auto f = cls_of.find(key);
-        int c;
-        if (f == cls_of.end()) {
-          c = (int)cls.size();
-          cls_of[key] = c;
-          cls.emplace_back();
-        } else {
-          c = f->second;
-        }
-        cls[c].push_back(&x);
-      }
-      for (auto& members : cls) {
-        double w = 0;
-        for (const HI* x : members) w += x->cost;
-        const int l = (int)(std::min_element(acc, acc + 8) - acc);
-        for (const HI* x : members) lane[l].push_back(x->it);
-        acc[l] += w;
-      }
-    } else {
-      std::map<int32_t, int> owner;
-      for (auto& x : all) {
-        if (x.large != large) continue;
-        auto o = owner.find(x.it.term);
-        int l;
-        if (o == owner.end()) {
-          l = (int)(std::min_element(acc, acc + 8) - acc);
-          owner[x.it.term] = l;
-        } else {
-          l = o->second;
-        }
-        lane[l].push_back(x.it);
-        acc[l] += x.cost;
-      }
-    }
-    size_t m = 0;
-    for (auto& v : lane) m = std::max(m, v.size());
-    Item pad{};
-    pad.term = -1;
-    for (size_t i = 0; i < m; ++i)
-      for (int l = 0; l < 8; ++l) h.push_back(i < lane[l].size() ? lane[l][i] : pad);
-    return h;
-  };
-  const std::vector<Item> h = order(false);
-  const std::vector<Item> h2 = order(true);
+  std::stable_sort(all.begin(), all.end(), [](const HItem& x, const HItem& y) { return x.cost > y.cost; });
+  const std::vector<Item> h = order_items(P, all, false, xcd);
+  const std::vector<Item> h2 = order_items(P, all, true, xcd);
   auto ip = std::make_unique<ItemPlan>();
   ip->nitems = (int64_t)h.size();
   ip->nbig = nbig;
@@ -752,6 +789,237 @@ static int grid_blocks(int64_t n) {
   return (int)std::min<int64_t>(std::max<int64_t>(b, 1), 8192);
 }
 
+static bool use_split(const sg4_plan* P);
+
+// bf16 limbs of the group matrices (round to nearest even)
+static uint16_t to_bf16(double d) {
+  const float f = (float)d;
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static double from_bf16(uint16_t h) {
+  const uint32_t u = (uint32_t)h << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// Terms and group matrices of the split-precision MFMA path.  The active
+// modes of a term are cut into groups of consecutive modes with Kronecker
+// size K <= 32 (the cut with the least padded work); the K x K matrices B_g = B_a (x) B_b (x) ... and
+// BtW_g (first index fastest, as the in-term index) are split into three
+// bf16 limbs and stored in the A-operand order of the lanes (limb, 16-row
+// tile, lane, slot: row 16 mt + lane%16, group index slot_k(lane/16, slot)),
+// zero-padded to 16 or 32.  Modes with equal 1-D matrices share group
+// matrices.
+static int build_mfma(sg4_plan* P) {
+  if (P->mfma_valid) return SG4_OK;
+  if (!use_split(P)) {
+    set_error("the split-precision MFMA mode needs the split type-1 path (square HO/Legendre "
+              "modes with exact quadrature, complete Smolyak term set)");
+    return SG4_ERR_UNSUPPORTED;
+  }
+  const size_t nmd = P->h_md.size();
+  std::vector<int> canon(nmd);
+  for (size_t i = 0; i < nmd; ++i) {
+    canon[i] = (int)i;
+    const MatDev& a = P->h_md[i];
+    const size_t nn = (size_t)a.nq * a.nb;
+    for (size_t q = 0; q < i; ++q) {
+      if (canon[q] != (int)q) continue;
+      const MatDev& b = P->h_md[q];
+      if (a.nq != b.nq || a.nb != b.nb) continue;
+      if (std::equal(&P->h_mats[a.B], &P->h_mats[a.B] + nn, &P->h_mats[b.B]) &&
+          std::equal(&P->h_mats[a.BtW], &P->h_mats[a.BtW] + nn, &P->h_mats[b.BtW])) {
+        canon[i] = (int)q;
+        break;
+      }
+    }
+  }
+  std::map<std::vector<int>, int> gid;
+  std::vector<uint16_t> tab;
+  auto put = [&](const std::vector<int>& key, bool btw) {
+    int K = 1;
+    for (int q : key) K *= P->h_md[q].nq;
+    const int S = (K <= 16) ? 4 : 8, MT = S / 4;
+    const size_t base = tab.size();
+    tab.resize(base + MAT_STRIDE, 0);
+    for (int mt = 0; mt < MT; ++mt)
+      for (int lane = 0; lane < 64; ++lane)
+        for (int sl = 0; sl < S; ++sl) {
+          const int row = 16 * mt + (lane & 15), j = lane >> 4;
+          const int k = (S == 4) ? 4 * j + sl : (sl < 4 ? 4 * j + sl : 12 + 4 * j + sl);
+          double a = 0.0;
+          if (row < K && k < K) {
+            a = 1.0;
+            int mr = row, kr = k;
+            for (int q : key) {
+              const MatDev& md = P->h_md[q];
+              const int n = md.nq;
+              a *= P->h_mats[(btw ? md.BtW : md.B) + (int64_t)(mr % n) * n + kr % n];
+              mr /= n;
+              kr /= n;
+            }
+          }
+          const uint16_t h = to_bf16(a);
+          const double r = a - from_bf16(h);
+          const uint16_t m = to_bf16(r);
+          const uint16_t l = to_bf16(r - from_bf16(m));
+          const uint16_t limb[3] = {h, m, l};
+          for (int li = 0; li < 3; ++li) tab[base + ((size_t)(li * MT + mt) * 64 + lane) * S + sl] = limb[li];
+        }
+  };
+  std::vector<TermM> tm(P->nterms);
+  for (int64_t t = 0; t < P->nterms; ++t) {
+    const TermDev& T = P->h_terms[t];
+    TermM M{};
+    M.boff = T.boff;
+    M.qoff = T.qoff;
+    M.wgt = T.wgt;
+    M.N = T.nq;
+    // cut the active modes into consecutive groups with K <= 32, minimising
+    // the per-point vector work of the passes: a 16-column tile costs about
+    // 47 vector instructions for K <= 16 (x16 products) and 73 for K <= 32
+    // (x32), spread over the K valid rows of the tile
+    const int na = T.nact;
+    std::vector<double> best(na + 1, 1e300);
+    std::vector<int> from(na + 1, -1);
+    best[0] = 0.0;
+    for (int i = 1; i <= na; ++i) {
+      int K = 1;
+      for (int q = i - 1; q >= 0; --q) {
+        K *= P->h_md[T.mat[q]].nq;
+        if (K > 32) break;
+        const double c = best[q] + (K <= 16 ? 47.0 : 73.0) / K;
+        if (c < best[i]) { best[i] = c; from[i] = q; }
+      }
+    }
+    if (na > 0 && from[na] < 0) {
+      set_error("a mode with more than 32 grid points has no MFMA tile");
+      return SG4_ERR_UNSUPPORTED;
+    }
+    std::vector<int> cut;
+    for (int i = na; i > 0; i = from[i]) cut.push_back(from[i]);
+    std::reverse(cut.begin(), cut.end());
+    cut.push_back(na);
+    int G = 0, A = 1;
+    for (size_t gi = 0; gi + 1 < cut.size(); ++gi) {
+      std::vector<int> key;
+      int K = 1;
+      for (int p = cut[gi]; p < cut[gi + 1]; ++p) {
+        K *= P->h_md[T.mat[p]].nq;
+        key.push_back(canon[T.mat[p]]);
+      }
+      if (G >= MG_MAX) { set_error("too many mode groups in a term"); return SG4_ERR_UNSUPPORTED; }
+      auto f = gid.find(key);
+      int idx;
+      if (f == gid.end()) {
+        idx = (int)(tab.size() / MAT_STRIDE);
+        put(key, false);
+        put(key, true);
+        gid[key] = idx;
+      } else {
+        idx = f->second;
+      }
+      M.A[G] = A;
+      M.K[G] = K;
+      M.mB[G] = idx;
+      ++G;
+      A *= K;
+    }
+    M.G = G;
+    tm[t] = M;
+  }
+  int rc;
+  if ((rc = P->d_termm.alloc(sizeof(TermM) * tm.size()))) return rc;
+  HIP_TRY(hipMemcpy(P->d_termm.p, tm.data(), sizeof(TermM) * tm.size(), hipMemcpyHostToDevice));
+  if ((rc = P->d_mtab.alloc(sizeof(uint16_t) * std::max<size_t>(tab.size(), 1)))) return rc;
+  if (!tab.empty())
+    HIP_TRY(hipMemcpy(P->d_mtab.p, tab.data(), sizeof(uint16_t) * tab.size(), hipMemcpyHostToDevice));
+  P->nmtab = (int64_t)(tab.size() / MAT_STRIDE);
+  P->h_termm.swap(tm);
+  P->mfma_valid = true;
+  return SG4_OK;
+}
+
+// Work items of the MFMA path: per term, chunks of nv = 2^g <= 16 vectors
+// with (4 RS + 8) N + 256 bytes of LDS (the FP32 tensor with row stride RS,
+// the FP32 potential, the map, scratch words) within the budget; a term that needs more than the budget with one
+// vector goes to the second list, launched with the full LDS.
+static int get_items_mfma(sg4_plan* P, int nvec, int budget, int xcd, ItemPlan** out) {
+  const int64_t key = -(((((int64_t)nvec * 1024 + budget / 1024) * 32 + xcd) * 2) + 1);
+  auto itp = P->items.find(key);
+  if (itp != P->items.end()) { *out = itp->second.get(); return SG4_OK; }
+  std::vector<HItem> all;
+  double flops_main = 0, flops_mfma = 0;
+  int lds = 0, lds2 = 0;
+  for (int64_t t = 0; t < P->nterms; ++t) {
+    const TermM& M = P->h_termm[t];
+    const TermDev& T = P->h_terms[t];
+    const int64_t N = M.G ? M.N : 0;
+    // LDS bytes: tensor (row stride RS >= nv floats), potential, map, 64 scratch words
+    auto need = [&](int g) { return (int64_t)(4 * mrow_stride(1 << g) + 8) * N + 256; };
+    if (need(0) > LDS_MAX) {
+      set_error("term with " + std::to_string(N) + " grid points exceeds the LDS of the MFMA path");
+      return SG4_ERR_UNSUPPORTED;
+    }
+    const bool large = need(0) > budget;
+    const int cap = large ? LDS_MAX : budget;
+    int gmax = 0;
+    while ((1 << (gmax + 1)) <= std::min(nvec, 16) && need(gmax + 1) <= cap) ++gmax;
+    double c1 = 0, fm1 = 0;   // per vector: cost model, MFMA flops
+    for (int g = 0; g < M.G; ++g) {
+      const double kp = M.K[g] <= 16 ? 16.0 : 32.0;
+      c1 += (double)N * (2.0 + 0.05 * kp * kp / M.K[g]);
+      fm1 += 2.0 * 6.0 * 2.0 * kp * kp * ((double)N / M.K[g]);
+    }
+    c1 = std::max(c1, 1.0);
+    double fa1 = 0;
+    for (int q = 0; q < T.nact; ++q) fa1 += 4.0 * (double)N * P->h_md[T.mat[q]].nq;
+    fa1 += (double)std::max<int64_t>(N, 1);
+    for (int v0 = 0; v0 < nvec;) {
+      int g = gmax;
+      while ((1 << g) > nvec - v0) --g;
+      HItem h{};
+      h.it.term = (int32_t)t;
+      h.it.v0 = (int16_t)v0;
+      h.it.lgnv = (int8_t)g;
+      h.cost = c1 * (1 << g);
+      h.large = large;
+      if (!large) {
+        flops_main += fa1 * (1 << g);
+        flops_mfma += fm1 * (1 << g);
+      }
+      int& l = large ? lds2 : lds;
+      l = std::max<int>(l, (int)need(g));
+      all.push_back(h);
+      v0 += 1 << g;
+    }
+  }
+  std::stable_sort(all.begin(), all.end(), [](const HItem& x, const HItem& y) { return x.cost > y.cost; });
+  const std::vector<Item> h = order_items(P, all, false, xcd);
+  const std::vector<Item> h2 = order_items(P, all, true, xcd);
+  auto ip = std::make_unique<ItemPlan>();
+  ip->nitems = (int64_t)h.size();
+  ip->lds = std::max(lds, 16);
+  ip->nitems2 = (int64_t)h2.size();
+  ip->lds2 = lds2;
+  ip->flops_main = flops_main;
+  ip->flops_mfma = flops_mfma;
+  int rc = ip->items.alloc(sizeof(Item) * std::max<size_t>(h.size(), 1));
+  if (rc) return rc;
+  if (!h.empty()) HIP_TRY(hipMemcpy(ip->items.p, h.data(), sizeof(Item) * h.size(), hipMemcpyHostToDevice));
+  if (!h2.empty()) {
+    if ((rc = ip->items2.alloc(sizeof(Item) * h2.size()))) return rc;
+    HIP_TRY(hipMemcpy(ip->items2.p, h2.data(), sizeof(Item) * h2.size(), hipMemcpyHostToDevice));
+  }
+  *out = ip.get();
+  P->items[key] = std::move(ip);
+  return SG4_OK;
+}
+
 static int ensure_f32(sg4_plan* P) {
   if (P->f32_valid) return SG4_OK;
   int rc;
@@ -772,9 +1040,9 @@ static int ensure_f32(sg4_plan* P) {
 static int check_apply(sg4_plan* P, int nvec, int prec) {
   if (P->type_op == 0) { set_error("no operator set on the plan"); return SG4_ERR_STATE; }
   if (nvec < 1 || nvec > P->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
-  if (prec != SG4_PREC_FP64 && prec != SG4_PREC_F32) {
-    set_error("precision mode not available");
-    return SG4_ERR_UNSUPPORTED;
+  if (prec != SG4_PREC_FP64 && prec != SG4_PREC_F32 && prec != SG4_PREC_F32X2) {
+    set_error("unknown precision mode");
+    return SG4_ERR_ARG;
   }
   if (P->type_op == 10 && prec != SG4_PREC_FP64) {
     set_error("the type-10 operator is available in FP64 only");
@@ -798,6 +1066,10 @@ static int prepare(sg4_plan* P, int nvec, int prec, const LaunchCfg& cfg, ItemPl
   int rc;
   const int esize = (prec == SG4_PREC_FP64) ? 8 : 4;
   if (esize == 4 && (rc = ensure_f32(P))) return rc;
+  if (prec == SG4_PREC_F32X2) {
+    if ((rc = build_mfma(P))) return rc;
+    return get_items_mfma(P, nvec, cfg.lds_budget, cfg.xcd, ip);
+  }
   const bool t10 = P->type_op == 10;
   return get_items(P, nvec, esize, ip, 0, t10 ? LDS_SMALL : cfg.lds_budget, t10 ? 0 : cfg.xcd,
                    use_split(P));
@@ -827,6 +1099,7 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
   a.slot_size = ip->slot_size;
   P->last_items = ip->nitems + ip->nitems2;
   P->last_flops = ip->flops_main;
+  P->last_flops_mfma = ip->flops_mfma;
   if (P->timing) {
     if (P->ev_used == P->ev_pairs.size()) {
       hipEvent_t e0, e1;
@@ -841,6 +1114,34 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
   }
   const unsigned grid = (unsigned)((cfg.grid > 0 && cfg.grid < ip->nitems) ? cfg.grid : ip->nitems);
   const dim3 blk(P->type_op == 10 ? 256 : (split ? cfg.nthreads : std::min(cfg.nthreads, NT)));
+  if (prec == SG4_PREC_F32X2) {
+    // kinetic rows in FP64 (initialise hpsi), then the terms on the matrix cores
+    hipLaunchKernelGGL(k_kin_rows, dim3(grid_blocks(n)), dim3(256), 0, P->stream,
+                       P->d_kptr.as<int64_t>(), P->d_kcol.as<int32_t>(), P->d_kval.as<double>(),
+                       psi, hpsi, P->nb_ba, nvec);
+    MArgs m;
+    m.terms = P->d_termm.as<TermM>();
+    m.mats = P->d_mtab.as<uint16_t>();
+    m.map = P->d_map.as<int32_t>();
+    m.V = P->d_V32.as<float>();
+    m.psi = psi;
+    m.hpsi = hpsi;
+    m.nvec = nvec;
+    for (int list = 0; list < 2; ++list) {
+      const int64_t ni = list ? ip->nitems2 : ip->nitems;
+      if (P->timing && list == 0) HIP_TRY(hipEventRecord(P->ev0, P->stream));
+      if (ni > 0) {
+        m.nitems = (int)ni;
+        m.items = list ? ip->items2.as<Item>() : ip->items.as<Item>();
+        const unsigned gr = (unsigned)((cfg.grid > 0 && cfg.grid < ni) ? cfg.grid : ni);
+        hipLaunchKernelGGL(k_hpsi_mfma, dim3(gr), dim3(std::min(cfg.nthreads, NT)),
+                           list ? ip->lds2 : ip->lds, P->stream, m);
+      }
+      if (P->timing && list == 0) HIP_TRY(hipEventRecord(P->ev1, P->stream));
+    }
+    HIP_TRY(hipGetLastError());
+    return SG4_OK;
+  }
   if (split) {
     // kinetic rows (initialise hpsi), then the potential part of the terms
     hipLaunchKernelGGL(k_kin_rows, dim3(grid_blocks(n)), dim3(256), 0, P->stream,
@@ -1011,6 +1312,8 @@ int sg4_plan_create_batch(int32_t ninst, const sg4_tables* tabs, int32_t max_nve
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
     (void)hipFuncSetAttribute((const void*)k_grad_v,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
+    (void)hipFuncSetAttribute((const void*)k_hpsi_mfma,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
   });
   std::unique_ptr<sg4_plan> P(new sg4_plan);
   P->max_nvec = max_nvec;
@@ -1169,6 +1472,7 @@ int sg4_plan_set_stream(sg4_plan* p, void* s) {
 
 int sg4_plan_set_operator(sg4_plan* p, const sg4_operator* op) {
   if (!p || !op || !op->V) { set_error("null operator"); return SG4_ERR_ARG; }
+  invalidate(p);
   int rc;
   if (op->type_op == 1) {
     if (!op->ck) { set_error("type 1 needs ck"); return SG4_ERR_ARG; }
@@ -1193,14 +1497,13 @@ int sg4_plan_set_operator(sg4_plan* p, const sg4_operator* op) {
   p->type_op = op->type_op;
   HIP_TRY(hipMemcpy(p->d_V.p, op->V, sizeof(double) * p->sum_nq, hipMemcpyHostToDevice));
   (void)p->d_V0.alloc(0);
-  p->items.clear();
-  p->f32_valid = false;
   return build_terms(p);
 }
 
 int sg4_plan_set_henon_heiles(sg4_plan* p, double lambda) {
   if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
   if (!p->have_nodes) { set_error("plan was created without grid nodes"); return SG4_ERR_STATE; }
+  invalidate(p);
   for (const sg4_plan::Inst& I : p->inst) {
     HHArgs a;
     a.levels = p->d_levels.as<int32_t>() + I.lev0;
@@ -1220,8 +1523,6 @@ int sg4_plan_set_henon_heiles(sg4_plan* p, double lambda) {
   int rc;
   if ((rc = p->d_ck.alloc(sizeof(double) * p->nck))) return rc;
   HIP_TRY(hipMemcpy(p->d_ck.p, p->h_ck.data(), sizeof(double) * p->nck, hipMemcpyHostToDevice));
-  p->items.clear();
-  p->f32_valid = false;
   rc = build_terms(p);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(p->stream));
@@ -1236,6 +1537,7 @@ int sg4_plan_set_jacobi_triatomic(sg4_plan* p, const double* prm) {
     return SG4_ERR_UNSUPPORTED;
   }
   if (!(prm[0] > 0) || !(prm[1] > 0)) { set_error("masses must be positive"); return SG4_ERR_ARG; }
+  invalidate(p);
   int rc;
   const size_t n = (size_t)p->sum_nq;
   if ((rc = p->d_G.alloc(sizeof(double) * n * 9))) return rc;
@@ -1260,8 +1562,6 @@ int sg4_plan_set_jacobi_triatomic(sg4_plan* p, const double* prm) {
   p->type_op = 10;
   (void)p->d_V0.alloc(0);
   p->h_ck.assign(p->nck, 0.0);
-  p->items.clear();
-  p->f32_valid = false;
   rc = build_terms(p);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(p->stream));
@@ -1318,6 +1618,13 @@ int sg4_plan_stats(const sg4_plan* p, int32_t nvec, double* flops, double* bytes
   if (items) {
     *items = p->last_items;
   }
+  return SG4_OK;
+}
+
+int sg4_plan_mfma_stats(const sg4_plan* p, double* mfma_flops, int64_t* group_mats) {
+  if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
+  if (mfma_flops) *mfma_flops = p->last_flops_mfma;
+  if (group_mats) *group_mats = p->mfma_valid ? p->nmtab : -1;
   return SG4_OK;
 }
 
@@ -1395,7 +1702,7 @@ int sg4_plan_autotune(sg4_plan* p, int32_t nvec, int32_t prec, float* best_ms) {
   }
   const int threads3[] = {256, 512, NT32};
   const std::vector<int> threads(threads3,
-                                 threads3 + ((use_split(p) && prec != SG4_PREC_FP64) ? 3 : 2));
+                                 threads3 + ((use_split(p) && prec == SG4_PREC_F32) ? 3 : 2));
   const int budgets[] = {32 * 1024, 48 * 1024, 64 * 1024, 80 * 1024, 112 * 1024, 160 * 1024};
   const int grids[] = {0, 4, 8};   // 0: one per item; k: k workgroups per CU, persistent
   for (int nt : threads)
@@ -1424,6 +1731,7 @@ int sg4_plan_autotune(sg4_plan* p, int32_t nvec, int32_t prec, float* best_ms) {
   (void)hipEventDestroy(e1);
   p->timing = timing;
   if (rc) return rc;
+  drop_graphs(p);
   p->tuned[nvec * 16 + prec] = best;
   if (best_ms) *best_ms = bms;
   return SG4_OK;
@@ -1439,7 +1747,7 @@ int sg4_plan_config(const sg4_plan* p, int32_t nvec, int32_t prec, int32_t* cfg4
 
 int sg4_plan_set_config(sg4_plan* p, int32_t nvec, int32_t prec, const int32_t* cfg4) {
   if (!p || !cfg4) { set_error("null argument"); return SG4_ERR_ARG; }
-  const bool t768 = cfg4[0] == NT32 && use_split(p) && prec != SG4_PREC_FP64;
+  const bool t768 = cfg4[0] == NT32 && use_split(p) && prec == SG4_PREC_F32;
   if ((cfg4[0] != 256 && cfg4[0] != 512 && !t768) || cfg4[1] < 1024 || cfg4[1] > LDS_MAX ||
       cfg4[2] < 0 || cfg4[3] < 0 || cfg4[3] > 8) {
     set_error("launch configuration out of range (threads 256/512, or 768 for the FP32 split "
@@ -1448,6 +1756,7 @@ int sg4_plan_set_config(sg4_plan* p, int32_t nvec, int32_t prec, const int32_t* 
   }
   LaunchCfg c;
   c.nthreads = cfg4[0]; c.lds_budget = cfg4[1]; c.grid = cfg4[2]; c.xcd = cfg4[3];
+  drop_graphs(p);
   p->tuned[nvec * 16 + prec] = c;
   return SG4_OK;
 }
@@ -1482,6 +1791,9 @@ int sg4_apply_graph(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, 
     (void)hipGraphDestroy(graph);
     p->graphs.push_back({nvec, prec, psi, hpsi, exec});
     g = &p->graphs.back();
+  } else if (prec != SG4_PREC_FP64 && (rc = ensure_f32(p))) {
+    // the FP32 potential is refreshed outside the graph after sg4_plan_set_field
+    return rc;
   }
   // order the replay after the caller's stream and the caller's stream after it
   hipEvent_t ev;
@@ -1499,8 +1811,8 @@ int sg4_plan_set_option(sg4_plan* p, int32_t option, int32_t value) {
   if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
   if (option != SG4_OPT_GROUPING) { set_error("unknown option"); return SG4_ERR_ARG; }
   p->grouping = value != 0;
+  invalidate(p);
   if (p->type_op == 0) return SG4_OK;
-  p->items.clear();
   return build_terms(p);
 }
 

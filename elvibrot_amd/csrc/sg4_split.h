@@ -117,13 +117,12 @@ __device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km
   const T* V = km.V + Td->qoff;
   const double w = Td->wgt;
   if (Td->nact == 0) {  // the one-point term
-    if ((int)threadIdx.x < nv) {
-      const int m = map[0];
-      if (m >= 0) {
-        const int64_t r = (int64_t)m * a.nvec + it.v0 + threadIdx.x;
+    const int m = map[0];
+    if (m >= 0)
+      for (int v = threadIdx.x; v < nv; v += blockDim.x) {
+        const int64_t r = (int64_t)m * a.nvec + it.v0 + v;
         atomicAdd(&a.hpsi[r], w * (double)(V[0] * (T)a.psi[r]));
       }
-    }
     return;
   }
   const int N = Td->nq;

@@ -75,12 +75,15 @@ class DavidsonResult:
 def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
                              conv_ene=1e-4, block=None, prec=PREC_FP64, psi0=None,
                              final_rr_fp64=True, device="cuda:0", verbose=False,
-                             switch_resi=None):
+                             switch_resi=None, refine=True):
     """Lowest ``nb_WP`` eigenpairs of the SG4 operator ``op``.
 
     Returns a DavidsonResult with E (host numpy), X (device, nb_WP x nb_ba),
     iterations, number of H.psi vector applications and, when ``prec`` is not
-    FP64, the FP64 Rayleigh-Ritz eigenvalues E_fp64 (E is then E_fp64)."""
+    FP64, the FP64 Rayleigh-Ritz eigenvalues E_fp64 (E is then E_fp64).
+    ``refine=False`` keeps the reduced precision to the end (no FP64 phase):
+    the eigenvalues are then those of the reduced-precision operator, the run
+    stopping at convergence or when the residuals stall at its noise floor."""
     nb = op.nb_ba
     block = block or min(op.max_nvec, nb_WP)
     max_WP = max_WP or max(4 * nb_WP, nb_WP + 4 * block)
@@ -166,7 +169,12 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
         if phase_prec != PREC_FP64:
             stall = stall + 1 if rn.max() > 0.7 * best_low else 0
             best_low = min(best_low, rn.max())
-        if phase_prec != PREC_FP64 and (rn.max() < switch_resi or stall >= 6):
+        if phase_prec != PREC_FP64 and not refine:
+            # no FP64 phase: run to convergence or until the residuals have
+            # not improved for 20 iterations (the reduced precision's floor)
+            if (conv.all() and it > 0) or stall >= 20:
+                break
+        elif phase_prec != PREC_FP64 and (rn.max() < switch_resi or stall >= 6):
             # FP64 phase: restart on the Ritz vectors with an FP64 H.psi
             n_low = n_hpsi
             keep = min(ndim, nb_WP + block)
@@ -195,7 +203,7 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
         Dg = Et - E0[None, :]
         Dg = torch.where(Dg.abs() > conv_resi, Dg, Dg + 1e-3)
         new = Rt / Dg
-    E_fp64 = E if prec != PREC_FP64 else None
+    E_fp64 = E if (prec != PREC_FP64 and refine) else None
     res = DavidsonResult(E, X, it, n_hpsi, conv, E_fp64, hist)
     res.n_hpsi_low = n_low      # H.psi vector applications done in the reduced precision
     return res

@@ -39,8 +39,12 @@ extern "C" {
 /* precision modes of sg4_apply */
 #define SG4_PREC_FP64 0      /* FP64 throughout (parity mode, 1e-12 relative) */
 #define SG4_PREC_F32 1       /* mode contractions in FP32, FP64 gather/scatter */
-#define SG4_PREC_F32X2 2     /* split FP32 (hi+lo) contractions on MFMA: reserved, applies
-                                 return SG4_ERR_UNSUPPORTED */
+#define SG4_PREC_F32X2 2     /* split-precision contractions on the matrix cores (split
+                                 type-1 path only): FP32 operands as three bf16 limbs
+                                 (hi + mid + lo), six limb products per product on
+                                 v_mfma_f32_16x16x{16,32}_bf16 with FP32 accumulation:
+                                 FP32-accurate contractions; FP64 kinetic part, term sum
+                                 and result */
 
 typedef struct sg4_setup sg4_setup; /* host tables built by this library */
 typedef struct sg4_plan sg4_plan;   /* device state of one SG4 operator */
@@ -227,6 +231,11 @@ int sg4_plan_split_info(const sg4_plan *p, int32_t *split, int64_t *kin_nnz);
  * map, one read of psi and one write of H.psi. */
 int sg4_plan_stats(const sg4_plan *p, int32_t nvec, double *flops, double *bytes,
                    int64_t *items);
+/* Split-precision mode (SG4_PREC_F32X2): flops the matrix cores executed in
+ * the last such apply's main term launch (padded Kronecker tiles times the
+ * six limb products) and the number of group matrices of the plan (-1 before
+ * the first such apply).  No counterpart in the reference. */
+int sg4_plan_mfma_stats(const sg4_plan *p, double *mfma_flops, int64_t *group_mats);
 /* Kernel timing for roofline reporting: when enabled, every apply records a
  * pair of HIP events on the plan's stream around its main H.psi kernel launch
  * (on the split path the term kernel's main item list, not the kinetic rows);

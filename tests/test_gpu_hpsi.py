@@ -335,3 +335,53 @@ def test_launch_configurations_keep_parity(cuda):
                 (512, 65536, -1, 0, E.PREC_FP64), (512, 65536, 0, 99, E.PREC_FP64)):
         with pytest.raises(E.SG4Error):
             op.set_config(nvec, bad[4], *bad[:4])
+
+
+@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_F32, E.PREC_F32X2])
+def test_block_wider_than_workgroup(cuda, prec):
+    """1024 vectors in one block: the one-point term's chunk holds more vectors
+    than a workgroup has threads, and every vector must still get its term."""
+    D, L, nvec = 6, 3, 1024
+    b, op, o, V = hh_pair(D, L, nvec)
+    psi = np.random.default_rng(77).standard_normal((b.nb_ba, nvec))
+    ref = o.hpsi(psi, V, -0.5 * np.ones(D))
+    got = op.apply_host(psi, prec)
+    err = rel(got, ref)
+    print(f"6D L=3 x1024 prec {prec}: {err:.2e}")
+    assert err < (TOL if prec == E.PREC_FP64 else 5e-6)
+
+
+def test_graph_recaptured_after_plan_changes(cuda):
+    """A captured graph points at plan buffers and was captured with one launch
+    configuration: after the grouping, the potential or the configuration
+    changes, sg4_apply_graph must give what sg4_apply_rows gives."""
+    import torch
+    D, L, nvec = 5, 4, 4
+    b = E.SG4Basis(D, L)
+    op = E.SG4Operator(b, max_nvec=nvec)
+    X = torch.randn(b.nb_ba, nvec, device=cuda, dtype=torch.float64,
+                    generator=torch.Generator(device=cuda).manual_seed(3))
+    Yg, Yr = torch.empty_like(X), torch.empty_like(X)
+
+    def same(prec=E.PREC_FP64, tol=1e-13):
+        op.apply_graph(X, Yg, prec)
+        op.apply_rows(X, Yr, prec)
+        torch.cuda.synchronize()
+        err = (torch.max(torch.abs(Yg - Yr)) / torch.max(torch.abs(Yr))).item()
+        assert err < tol, err
+
+    same()
+    op.set_grouping(False)
+    same()
+    op.set_grouping(True)
+    same()
+    E._capi.check(E.lib().sg4_plan_set_henon_heiles(op._h, 0.2))
+    same()
+    op.set_config(nvec, E.PREC_FP64, nthreads=256, lds_budget=32768, grid=0, xcd=1)
+    same()
+    # FP32 replay after a field change reads the refreshed FP32 potential
+    mu = torch.ones(b.sum_nq, device=cuda, dtype=torch.float64)
+    same(E.PREC_F32, 1e-6)
+    op.set_field(mu, 0.3)
+    same(E.PREC_F32, 1e-6)
+    op.set_field(None, 0.0)

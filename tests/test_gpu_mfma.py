@@ -1,0 +1,142 @@
+"""Split-precision MFMA mode (SG4_PREC_F32X2) on the MI355X through the C ABI.
+
+The contractions run on the matrix cores with three bf16 limbs per FP32
+operand (six products, FP32 accumulation), each as accurate as an FP32
+contraction; the kinetic part, the sum over terms and the result are FP64.
+The Smolyak combination amplifies the per-term rounding: the potential
+parts W_t r_t of the terms sum to H.psi with cancellations of up to
+max_e sum_t |W_t r_t| / max|H psi| ~ 2e2 (6D L=6) and 7e2 (10D L=5).  Stated
+tolerance of the mode against the FP64 oracle, checked below:
+max|got - ref| <= 8 eps32 max_e sum_t |W_t r_t| (the oracle computes the
+bound), and a 2-norm relative error <= 2e-6.
+"""
+import numpy as np
+import pytest
+
+import elvibrot_amd as E
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL2 = 2e-6                               # 2-norm relative
+EPS32 = float(np.finfo(np.float32).eps)
+KCOND = 8.0                               # max-norm: KCOND eps32 max_e sum_t |W_t r_t|
+
+
+def errs(got, ref):
+    return (np.linalg.norm(got - ref) / np.linalg.norm(ref),
+            np.max(np.abs(got - ref)) / np.max(np.abs(ref)))
+
+
+def cond_ratio(got, ref, o, psi, V, D):
+    """max|got - ref| in units of eps32 max_e sum_t |W_t r_t| (potential parts)."""
+    bound = o.absbound(psi, V, np.zeros(D))
+    return np.max(np.abs(got - ref)) / (EPS32 * np.max(bound))
+
+
+@pytest.mark.parametrize("D,L,nvec", [(6, 3, 4), (6, 6, 16), (10, 5, 16), (4, 5, 7), (3, 10, 5),
+                                      (2, 10, 16), (21, 2, 3)])
+def test_f32x2_parity_vs_oracle(cuda, D, L, nvec):
+    print(f"case D={D} L={L} nvec={nvec}", flush=True)
+    b = E.SG4Basis(D, L)
+    op = E.SG4Operator(b, max_nvec=16, potential=("henon_heiles", 0.111803))
+    assert op.split_info()[0]
+    o = O.SG4(D, L)
+    V = o.grid_hh(0.111803)
+    psi = np.random.default_rng(100 * D + L).standard_normal((b.nb_ba, nvec))
+    ref = o.hpsi(psi, V, -0.5 * np.ones(D))
+    got = op.apply_host(psi, E.PREC_F32X2)
+    e2, em = errs(got, ref)
+    rc = cond_ratio(got, ref, o, psi, V, D)
+    f, ngm = op.mfma_stats()
+    print(f"D={D} L={L}: 2-norm {e2:.2e} max {em:.2e}, max / (eps32 bound) {rc:.3f}; "
+          f"group matrices {ngm}, MFMA flops {f:.3g}")
+    assert e2 < TOL2 and rc < KCOND
+    assert ngm > 0 and f > 0
+
+
+def test_f32x2_general_potential_absent_rows(cuda):
+    # arbitrary grid values, unequal kinetic coefficients, LB < LG (absent map rows)
+    D, L, LB = 4, 5, 3
+    b = E.SG4Basis(D, L, LB)
+    o = O.SG4(D, L, LB)
+    rng = np.random.default_rng(11)
+    V = rng.standard_normal(o.sum_nq)
+    ck = np.array([-0.5, -0.25, -1.0, -0.75])
+    op = E.SG4Operator(b, 8, potential=V, ck=ck)
+    if not op.split_info()[0]:
+        pytest.skip("this basis does not take the split path")
+    psi = rng.standard_normal((b.nb_ba, 6))
+    got, ref = op.apply_host(psi, E.PREC_F32X2), o.hpsi(psi, V, ck)
+    e2, em = errs(got, ref)
+    rc = cond_ratio(got, ref, o, psi, V, D)
+    print(f"4D L=5 LB=3 random V: 2-norm {e2:.2e} max {em:.2e} ratio {rc:.3f}")
+    assert e2 < TOL2 and rc < KCOND
+
+
+def test_f32x2_10d_L8_block16_against_fp64(cuda):
+    """The headline workload: every vector of the 16-block against the FP64
+    apply (itself at 1e-12 of the oracle) in the 2-norm, two of them against
+    the oracle with the conditioning bound of the max norm, symmetry of the
+    projected matrix, and linearity."""
+    import torch
+    D, L, nvec = 10, 8, 16
+    b = E.SG4Basis(D, L)
+    op = E.SG4Operator(b, max_nvec=nvec)
+    g = torch.Generator(device=cuda).manual_seed(2026)
+    X = torch.randn(b.nb_ba, nvec, device=cuda, dtype=torch.float64, generator=g)
+    X, _ = torch.linalg.qr(X)
+    Xc = X.T.contiguous()
+    H64 = torch.empty_like(Xc)
+    H32 = torch.empty_like(Xc)
+    op.apply(Xc, H64, E.PREC_FP64)
+    op.apply(Xc, H32, E.PREC_F32X2)
+    torch.cuda.synchronize()
+    e2 = (torch.linalg.norm(H32 - H64, dim=1) / torch.linalg.norm(H64, dim=1)).max().item()
+    em = (torch.max(torch.abs(H32 - H64)) / torch.max(torch.abs(H64))).item()
+    S = Xc @ H32.T
+    sym = (torch.max(torch.abs(S - S.T)) / torch.max(torch.abs(S))).item()
+    Y = (2 * Xc[0:1] - 3 * Xc[1:2]).contiguous()
+    HY = torch.empty_like(Y)
+    op.apply(Y, HY, E.PREC_F32X2)
+    lin = 2 * H32[0:1] - 3 * H32[1:2]
+    el = (torch.linalg.norm(HY - lin) / torch.linalg.norm(lin)).item()
+    o = O.SG4(D, L)
+    V = o.grid_hh(0.111803)
+    psi = Xc[:2].T.cpu().numpy()
+    got = H32[:2].T.cpu().numpy()
+    ref = o.hpsi(psi, V, -0.5 * np.ones(D))
+    rc = cond_ratio(got, ref, o, psi, V, D)
+    print(f"10D L=8 x16 split vs FP64: worst vector 2-norm {e2:.2e}, max {em:.2e}, "
+          f"symmetry {sym:.2e}, linearity {el:.2e}; 2 vectors vs oracle: max / (eps32 bound) "
+          f"{rc:.3f}")
+    assert e2 < TOL2 and rc < KCOND
+    assert sym < TOL2 and el < TOL2
+
+
+def test_f32x2_repeatable_and_graph(cuda):
+    """Two applies differ only by the order of the FP64 atomic additions; a
+    captured graph of the split mode equals the eager apply to that level."""
+    import torch
+    D, L, nvec = 6, 5, 8
+    b = E.SG4Basis(D, L)
+    op = E.SG4Operator(b, max_nvec=nvec)
+    X = torch.randn(b.nb_ba, nvec, device=cuda, dtype=torch.float64,
+                    generator=torch.Generator(device=cuda).manual_seed(5))
+    Y1, Y2, Yg = (torch.empty_like(X) for _ in range(3))
+    op.apply_rows(X, Y1, E.PREC_F32X2)
+    op.apply_rows(X, Y2, E.PREC_F32X2)
+    for _ in range(3):
+        op.apply_graph(X, Yg, E.PREC_F32X2)
+        torch.cuda.synchronize()
+        assert (torch.max(torch.abs(Yg - Y1)) / torch.max(torch.abs(Y1))).item() < 1e-12
+    assert (torch.max(torch.abs(Y2 - Y1)) / torch.max(torch.abs(Y1))).item() < 1e-12
+
+
+def test_f32x2_rejected_off_the_split_path(cuda):
+    # one pass per mode (grouping off) is not the split path: the mode is refused
+    b = E.SG4Basis(4, 4)
+    op = E.SG4Operator(b, max_nvec=2)
+    op.set_grouping(False)
+    with pytest.raises(E.SG4Error):
+        op.apply_host(np.ones((b.nb_ba, 2)), E.PREC_F32X2)
