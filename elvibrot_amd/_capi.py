@@ -21,7 +21,7 @@ EXPORTS = [
     "sg4_apply_host", "sg4_apply_rows", "sg4_plan_stats", "sg4_plan_timing",
     "sg4_plan_kernel_times", "sg4_grad_V", "sg4_plan_set_field", "sg4_plan_autotune", "sg4_plan_config", "sg4_plan_set_config",
     "sg4_apply_graph", "sg4_plan_set_option", "sg4_plan_split_info", "sg4_plan_mfma_stats",
-    "sg4_plan_destroy",
+    "sg4_plan_destroy", "sg4_apply_noalloc",
 ]
 
 
@@ -69,7 +69,7 @@ def lib():
         L.sg4_plan_set_stream.argtypes = [vp, vp]
         L.sg4_plan_set_operator.argtypes = [vp, C.POINTER(Operator)]
         L.sg4_plan_set_henon_heiles.argtypes = [vp, C.c_double]
-        for f in ("sg4_apply", "sg4_apply_rows"):
+        for f in ("sg4_apply", "sg4_apply_rows", "sg4_apply_noalloc"):
             getattr(L, f).argtypes = [vp, i32, vp, vp, i32]
         L.sg4_apply_host.argtypes = [vp, i32, D_P, D_P, i32]
         L.sg4_plan_stats.argtypes = [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_double),

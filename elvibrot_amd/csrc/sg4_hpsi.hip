@@ -277,6 +277,7 @@ struct sg4_plan {
   double last_flops = 0;     // algorithmic flops of the last apply's main H.psi launch
   double last_flops_mfma = 0;  // MFMA path: flops the matrix cores executed in that launch
   bool grouping = true;      // register-blocked group passes for square terms
+  bool no_alloc = false;     // inside sg4_apply_noalloc: use prepared state only
   // split type-1 path: kinetic sparse matrix K (CSR over packed rows) and the
   // host copies of the map and matrices it is assembled from
   bool split_ok = false;
@@ -303,6 +304,14 @@ struct sg4_plan {
 };
 
 namespace sg4 {
+
+// The allocation-free entry point found no prepared state for its call.
+static int not_prepared() {
+  set_error("sg4_apply_noalloc: the plan holds no work items (or FP32 / matrix-core tables) "
+            "for this block size, precision and launch configuration; one sg4_apply call "
+            "with them prepares the plan");
+  return SG4_ERR_STATE;
+}
 
 // Captured graphs hold kernel arguments that point at plan buffers (item
 // lists, the term and matrix tables, the kinetic matrix, the FP32 copies) and
@@ -684,6 +693,7 @@ static int get_items(sg4_plan* P, int nvec, int esize, ItemPlan** out, int nbuf_
       (split ? 1 : 0);
   auto itp = P->items.find(key);
   if (itp != P->items.end()) { *out = itp->second.get(); return SG4_OK; }
+  if (P->no_alloc) return not_prepared();
   std::vector<HItem> all;
   double flops_main = 0;
   all.reserve(P->nterms * 2);
@@ -816,6 +826,7 @@ static double from_bf16(uint16_t h) {
 // matrices.
 static int build_mfma(sg4_plan* P) {
   if (P->mfma_valid) return SG4_OK;
+  if (P->no_alloc) return not_prepared();
   if (!use_split(P)) {
     set_error("the split-precision MFMA mode needs the split type-1 path (square HO/Legendre "
               "modes with exact quadrature, complete Smolyak term set)");
@@ -952,6 +963,7 @@ static int get_items_mfma(sg4_plan* P, int nvec, int budget, int xcd, ItemPlan**
   const int64_t key = -(((((int64_t)nvec * 1024 + budget / 1024) * 32 + xcd) * 2) + 1);
   auto itp = P->items.find(key);
   if (itp != P->items.end()) { *out = itp->second.get(); return SG4_OK; }
+  if (P->no_alloc) return not_prepared();
   std::vector<HItem> all;
   double flops_main = 0, flops_mfma = 0;
   int lds = 0, lds2 = 0;
@@ -1022,6 +1034,7 @@ static int get_items_mfma(sg4_plan* P, int nvec, int budget, int xcd, ItemPlan**
 
 static int ensure_f32(sg4_plan* P) {
   if (P->f32_valid) return SG4_OK;
+  if (P->no_alloc && (!P->d_mats32.p || !P->d_V32.p || !P->d_ck32.p)) return not_prepared();
   int rc;
   if (!P->d_mats32.p && (rc = P->d_mats32.alloc(sizeof(float) * P->nmats))) return rc;
   if (!P->d_V32.p && (rc = P->d_V32.alloc(sizeof(float) * P->sum_nq))) return rc;
@@ -1587,6 +1600,65 @@ int sg4_apply(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_
   hipLaunchKernelGGL(k_cols_to_rows, tile_grid(p->nb_ba, nvec), dim3(256), 0, p->stream, psi, ri, p->nb_ba, nvec);
   int rc = apply_rows(p, nvec, ri, ro, prec);
   if (rc) return rc;
+  hipLaunchKernelGGL(k_rows_to_cols, tile_grid(p->nb_ba, nvec), dim3(256), 0, p->stream, ro, hpsi, p->nb_ba, nvec);
+  HIP_TRY(hipGetLastError());
+  return SG4_OK;
+}
+
+}  // extern "C"
+
+// Allocation-free entry point: the two rows-layout staging blocks of
+// sg4_apply live in this file-scope device array (one copy per GPU, held
+// until the process ends).  In host code `scratch` only names the symbol:
+// its device address on the current GPU comes from hipGetSymbolAddress,
+// looked up on the first call on that GPU and kept per device.
+static constexpr size_t SG4_SCRATCH_BYTES = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[SG4_SCRATCH_BYTES];
+static constexpr int SG4_MAX_GPUS = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[SG4_MAX_GPUS] = {};
+
+static int scratch_address(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SG4_MAX_GPUS) { set_error("device index out of range"); return SG4_ERR_ARG; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return SG4_OK;
+}
+
+extern "C" {
+
+int sg4_apply_noalloc(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_t prec) {
+  if (!p || !psi || !hpsi) { set_error("null argument"); return SG4_ERR_ARG; }
+  if (nvec < 1 || nvec > p->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
+  if (blocks_overlap(p, nvec, psi, hpsi)) return SG4_ERR_ARG;
+  const size_t n = (size_t)p->nb_ba * nvec;
+  if (nvec > 1 && 2 * sizeof(double) * n > SG4_SCRATCH_BYTES) {
+    set_error("sg4_apply_noalloc: nb_ba x nvec = " + std::to_string(n) +
+              " exceeds its device scratch (two FP64 blocks in 256 MiB: nb_ba x nvec <= 16777216)");
+    return SG4_ERR_ARG;
+  }
+  double* base = nullptr;
+  int rc;
+  if (nvec > 1 && (rc = scratch_address((void**)&base))) return rc;
+  // prepared state only, and no timing events (creating them allocates)
+  struct Guard {
+    sg4_plan* p;
+    bool timing;
+    explicit Guard(sg4_plan* q) : p(q), timing(q->timing) { p->no_alloc = true; p->timing = false; }
+    ~Guard() { p->no_alloc = false; p->timing = timing; }
+  } guard(p);
+  if (nvec == 1) return apply_rows(p, 1, psi, hpsi, prec);
+  double* ri = base;
+  double* ro = base + n;
+  hipLaunchKernelGGL(k_cols_to_rows, tile_grid(p->nb_ba, nvec), dim3(256), 0, p->stream, psi, ri, p->nb_ba, nvec);
+  if ((rc = apply_rows(p, nvec, ri, ro, prec))) return rc;
   hipLaunchKernelGGL(k_rows_to_cols, tile_grid(p->nb_ba, nvec), dim3(256), 0, p->stream, ro, hpsi, p->nb_ba, nvec);
   HIP_TRY(hipGetLastError());
   return SG4_OK;

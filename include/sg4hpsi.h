@@ -162,6 +162,20 @@ int sg4_plan_set_jacobi_triatomic(sg4_plan *p, const double *prm);
  * reference): overlapping blocks are rejected with SG4_ERR_ARG.  The same holds
  * for sg4_apply_rows and sg4_apply_graph. */
 int sg4_apply(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);
+/* Allocation-free variant of sg4_apply, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  It calls no hipMalloc,
+ * hipFree or allocating library routine: the two rows-layout staging blocks
+ * live in a file-scope device array of 256 MiB per GPU (held until the
+ * process ends), so nb_ba x nvec <= 16777216 (larger blocks: SG4_ERR_ARG),
+ * and it only uses work items, FP32 copies and matrix-core tables that an
+ * earlier sg4_apply with the same nvec, precision and launch configuration
+ * has prepared (otherwise SG4_ERR_STATE).  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (several streams or threads)
+ * race and must be serialised by the caller.  Nothing in the array needs to
+ * start at zero (each call overwrites what it reads).  Kernel timing
+ * (sg4_plan_timing) is not recorded by this entry point. */
+int sg4_apply_noalloc(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);
 /* Same with HOST pointers (copies in and out over PCIe; synchronous). */
 int sg4_apply_host(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);
 /* Same in the interleaved device layout used by the device-side drivers:
