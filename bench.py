@@ -96,6 +96,63 @@ def measured_traffic(kernel, D, L, nvec):
     return float(m.group(1)) if m else None
 
 
+def propagation_bench(E, torch, args):
+    """64 uneven instances (D ~ U{4..12}, L ~ U{4..8}, numpy seed 7) in one
+    batched plan, dipole sum_i q_i, 200 piecewise-constant N(0, 0.01)
+    amplitudes (seed 7), steps of dt = 0.1 spread evenly over the segments;
+    forward propagation of one complex state per instance with the device
+    propagator, then J and dJ/deps over the first segments."""
+    from elvibrot_amd.propagation import (ChebyshevPropagator, spectral_bounds, propagate_device,
+                                          field_gradient_device)
+    prec = PREC[args.prec]
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(7)
+    Ds = rng.integers(4, 13, size=64)
+    Ls = rng.integers(4, 9, size=64)
+    bases = [E.SG4Basis(int(d), int(l)) for d, l in zip(Ds, Ls)]
+    op = E.SG4Batch(bases, max_nvec=4)
+    eps = np.random.default_rng(7).normal(0.0, 0.01, 200)
+    mu = op.grid_dipole()
+    setup = time.perf_counter() - t0
+    for nv in (2, 4):
+        op.autotune(nv, prec)
+        if prec != PREC["fp64"]:
+            op.autotune(nv, PREC["fp64"])
+    lo, hi = spectral_bounds(op)
+    w = 0.05 * (hi - lo) + np.abs(eps).max() * mu.abs().max().item()
+    prop = ChebyshevPropagator(op, 0.1, lo - w, hi + w, mu=mu, prec=prec)
+    nb = op.basis.nb_ba
+    blocks = list(zip(op.row0[:-1], op.row0[1:]))
+    psi0 = torch.zeros(1, nb, dtype=torch.complex128, device="cuda")
+    target = torch.zeros_like(psi0)
+    for r0, r1 in blocks:
+        psi0[0, r0] = 1.0
+        target[0, r0] = np.sqrt(0.5)
+        target[0, r0 + 1] = np.sqrt(0.5)
+    spp = max(1, args.propagation // len(eps))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    psiT = propagate_device(prop, psi0, eps, spp)
+    torch.cuda.synchronize()
+    tf = time.perf_counter() - t1
+    nsteps = spp * len(eps)
+    gseg = max(1, args.grad_steps // spp)
+    t2 = time.perf_counter()
+    J, g = field_gradient_device(prop, psi0, target, eps[:gseg], spp, checkpoint_every=5,
+                                 blocks=blocks)
+    tg = time.perf_counter() - t2
+    napp = nsteps * (prop.nmax - 1)
+    return {"workload": "64 uneven Henon-Heiles instances (seed 7), 200-segment field, "
+                        f"{nsteps} Chebyshev steps of dt = 0.1",
+            "precision": args.prec, "rows": nb, "grid_points": op.basis.sum_nq,
+            "setup_s": setup, "spectral_range": [lo, hi], "orders_per_step": prop.nmax,
+            "forward_s": tf, "steps_per_s": nsteps / tf,
+            "batch_hpsi_per_s": napp / tf, "norm_after": torch.linalg.norm(psiT).item() / 8.0,
+            "gradient": {"segments": gseg, "steps": gseg * spp, "seconds": tg, "J": J,
+                         "dJ_deps_first": [float(x) for x in g[:4]]},
+            "launch": {"nvec2": op.config(2, prec), "nvec4": op.config(4, prec)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +173,12 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (x5), for scaling HBM counters")
     ap.add_argument("--no-other", action="store_true", help="do not time the other precision")
+    ap.add_argument("--propagation", type=int, default=0, metavar="NSTEPS",
+                    help="only time the batched propagation workload: 64 seeded uneven "
+                         "Henon-Heiles instances, 200-segment field, NSTEPS Chebyshev steps of "
+                         "dt = 0.1 (forward), plus the field gradient over the first "
+                         "--grad-steps steps")
+    ap.add_argument("--grad-steps", type=int, default=50)
     ap.add_argument("--D", type=int, default=10)
     ap.add_argument("--L", type=int, default=8)
     ap.add_argument("--nvec", type=int, default=16)
@@ -139,6 +202,10 @@ def main():
             c.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes_read": n * 8, "bytes_written": n * 8, "launches": 5}))
+        return
+
+    if args.propagation:
+        print(json.dumps(propagation_bench(E, torch, args)), flush=True)
         return
 
     D, L, nvec = args.D, args.L, args.nvec

@@ -263,6 +263,13 @@ class SG4Operator:
         _capi.check(lib().sg4_plan_kernel_times(self._h, buf, max_n, C.byref(n)))
         return np.array(buf[: n.value], dtype=np.float64)
 
+    def grid_dipole(self):
+        """mu = sum_k q_k on every grid point (device tensor of sum_nq values)."""
+        import torch
+        mu = torch.empty(self.basis.sum_nq, dtype=torch.float64, device="cuda")
+        _capi.check(lib().sg4_plan_grid_dipole(self._h, C.c_void_p(mu.data_ptr())))
+        return mu
+
     def mfma_stats(self):
         """(flops the matrix cores executed in the last split-precision apply's
         term launch, number of group matrices; -1 before the first such apply)"""

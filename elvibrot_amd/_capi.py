@@ -21,7 +21,8 @@ EXPORTS = [
     "sg4_apply_host", "sg4_apply_rows", "sg4_plan_stats", "sg4_plan_timing",
     "sg4_plan_kernel_times", "sg4_grad_V", "sg4_plan_set_field", "sg4_plan_autotune", "sg4_plan_config", "sg4_plan_set_config",
     "sg4_apply_graph", "sg4_plan_set_option", "sg4_plan_split_info", "sg4_plan_mfma_stats",
-    "sg4_plan_destroy", "sg4_apply_noalloc",
+    "sg4_plan_destroy", "sg4_apply_noalloc", "sg4_cheby_propagate", "sg4_cheby_field_gradient",
+    "sg4_plan_grid_dipole",
 ]
 
 
@@ -39,6 +40,13 @@ class Tables(C.Structure):
     _fields_ = [("D", C.c_int32), ("Lmax", C.c_int32), ("nterms", C.c_int64),
                 ("nb_ba", C.c_int64), ("levels", I_P), ("weights", D_P), ("map", I_P),
                 ("nq", I_P), ("nb", I_P), ("mats", D_P), ("nodes", D_P), ("qweights", D_P)]
+
+
+class ChebyParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("E0", C.c_double), ("Esc", C.c_double),
+                ("ncoef", C.c_int32), ("coef", D_P), ("nseg", C.c_int32),
+                ("steps_per_seg", C.c_int32), ("eps", D_P), ("ckpt_every", C.c_int32),
+                ("prec", C.c_int32)]
 
 
 class Operator(C.Structure):
@@ -88,6 +96,11 @@ def lib():
         L.sg4_plan_split_info.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_int64)]
         L.sg4_plan_mfma_stats.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.sg4_plan_destroy.argtypes = [vp]
+        L.sg4_plan_grid_dipole.argtypes = [vp, vp]
+        L.sg4_cheby_propagate.argtypes = [vp, vp, i32, vp, C.POINTER(ChebyParams)]
+        L.sg4_cheby_field_gradient.argtypes = [vp, vp, vp, i32, vp, vp, i32,
+                                               C.POINTER(C.c_int64), C.POINTER(ChebyParams),
+                                               C.POINTER(C.c_double), D_P]
         _lib = L
     return _lib
 

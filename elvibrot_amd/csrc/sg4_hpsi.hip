@@ -147,6 +147,25 @@ __global__ void k_grid_hh(HHArgs a) {
   }
 }
 
+// Dipole mu = sum_k q_k at every grid point of every term (first index fastest).
+__global__ void k_grid_dipole(HHArgs a) {
+  for (int64_t t = blockIdx.x; t < a.nterms; t += gridDim.x) {
+    const int64_t q0 = a.qoff[t], nqt = a.qoff[t + 1] - q0;
+    for (int64_t e = threadIdx.x; e < nqt; e += blockDim.x) {
+      int64_t r = e;
+      double m = 0.0;
+      for (int k = 0; k < a.D; ++k) {
+        const int l = a.levels[(int64_t)t * a.D + k];
+        const int idx = k * (a.Lmax + 1) + l;
+        const int n = a.nqtab[idx];
+        m += a.nodes[a.node_off[idx] + (int)(r % n)];
+        r /= n;
+      }
+      a.V[q0 + e] = m;
+    }
+  }
+}
+
 // Triatomic model in Jacobi coordinates (R, r, theta), J = 0, type-10
 // operator: V = D(1 - e^{-a(R-Re)})^2 + D(1 - e^{-a(r-re)})^2 + c cos^2 theta,
 // G = diag(1/mu_R, 1/mu_r, 1/(mu_R R^2) + 1/(mu_r r^2)), Jac = sin theta,
@@ -304,6 +323,10 @@ struct sg4_plan {
 };
 
 namespace sg4 {
+
+void* plan_stream(const sg4_plan* p) { return p->stream; }
+int64_t plan_rows(const sg4_plan* p) { return p->nb_ba; }
+int plan_max_nvec(const sg4_plan* p) { return p->max_nvec; }
 
 // The allocation-free entry point found no prepared state for its call.
 static int not_prepared() {
@@ -1539,6 +1562,25 @@ int sg4_plan_set_henon_heiles(sg4_plan* p, double lambda) {
   rc = build_terms(p);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(p->stream));
+  return SG4_OK;
+}
+
+int sg4_plan_grid_dipole(sg4_plan* p, double* mu) {
+  if (!p || !mu) { set_error("null argument"); return SG4_ERR_ARG; }
+  if (!p->have_nodes) { set_error("plan was created without grid nodes"); return SG4_ERR_STATE; }
+  for (const sg4_plan::Inst& I : p->inst) {
+    HHArgs a;
+    a.levels = p->d_levels.as<int32_t>() + I.lev0;
+    a.qoff = p->d_qoff.as<int64_t>() + I.term0;
+    a.nodes = p->d_nodes.as<double>();
+    a.node_off = p->d_node_off.as<int64_t>() + I.md0;
+    a.nqtab = p->d_nqtab.as<int32_t>() + I.md0;
+    a.D = I.D; a.Lmax = I.Lmax; a.nterms = I.nterms; a.lambda = 0.0;
+    a.V = mu;
+    hipLaunchKernelGGL(k_grid_dipole, dim3((unsigned)std::min<int64_t>(I.nterms, 65535)), dim3(256),
+                       0, p->stream, a);
+  }
+  HIP_TRY(hipGetLastError());
   return SG4_OK;
 }
 

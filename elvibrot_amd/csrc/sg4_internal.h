@@ -46,6 +46,11 @@ struct Setup {
 
 void set_error(const std::string& msg);
 
+// plan accessors for the propagator translation unit
+void* plan_stream(const sg4_plan* p);
+int64_t plan_rows(const sg4_plan* p);
+int plan_max_nvec(const sg4_plan* p);
+
 }  // namespace sg4
 
 struct sg4_setup {

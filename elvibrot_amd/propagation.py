@@ -25,13 +25,15 @@ and Esc fixed), and dJ/deps_s = 2 Re(conj(a) <lambda_s | dpsi_s/deps_s>) with
 a = <target|psi(T)>.  States are checkpointed every `checkpoint_every`
 segments; the backward sweep recomputes the segments between checkpoints.
 """
+import ctypes as C
 import math
 
 import numpy as np
 import torch
 from scipy.special import jv
 
-from . import PREC_FP64, SG4Operator, SG4Batch
+from . import PREC_FP64, SG4Operator, SG4Batch, _capi
+from ._capi import lib
 from .psi_io import write_restart
 
 
@@ -248,3 +250,68 @@ def overlap_gradient(prop, psi0, target, eps, steps_per_segment, checkpoint_ever
             lam, _ = prop.step(lam, conj=True, npoly=npolys[s][j])
     prop.set_field(0.0)
     return J, grad
+
+
+# ---- device propagator (sg4_cheby_propagate / sg4_cheby_field_gradient) ----
+def to_rows(x):
+    """complex (ns, nb_ba) -> the C ABI's rows layout: float64 (nb_ba, 2 ns),
+    row r = real parts of the ns states, then their imaginary parts."""
+    return torch.cat([x.real, x.imag], 0).T.contiguous()
+
+
+def from_rows(r, ns):
+    return torch.complex(r[:, :ns].T.contiguous(), r[:, ns:].T.contiguous())
+
+
+def _params(prop, eps, steps_per_segment, checkpoint_every=10):
+    coef = np.ascontiguousarray(prop.coef[:prop.nmax], dtype=np.float64)
+    ev = np.ascontiguousarray(eps, dtype=np.float64)
+    prm = _capi.ChebyParams()
+    prm.dt, prm.E0, prm.Esc = prop.dt, prop.E0, prop.Esc
+    prm.ncoef = len(coef)
+    prm.coef = coef.ctypes.data_as(_capi.D_P)
+    prm.nseg = len(ev)
+    prm.steps_per_seg = steps_per_segment
+    prm.eps = ev.ctypes.data_as(_capi.D_P)
+    prm.ckpt_every = checkpoint_every
+    prm.prec = prop.prec
+    return prm, (coef, ev)
+
+
+def propagate_device(prop, psi0, eps, steps_per_segment):
+    """psi(T) through all segments with the device propagator: the same map as
+    propagate_field with the fixed order count prop.nmax, the recurrence in
+    device kernels with one H.psi launch per order and no host synchronisation
+    per order."""
+    ns = psi0.shape[0]
+    x = to_rows(psi0)
+    prm, keep = _params(prop, eps, steps_per_segment)
+    mu = prop.mu_grid.contiguous()
+    _capi.check(lib().sg4_cheby_propagate(prop.op._h, C.c_void_p(mu.data_ptr()), ns,
+                                          C.c_void_p(x.data_ptr()), C.byref(prm)))
+    return from_rows(x, ns)
+
+
+def field_gradient_device(prop, psi0, target, eps, steps_per_segment, checkpoint_every=10,
+                          blocks=None):
+    """J = sum_i |<target_i|psi_i(T)>|^2 over the row blocks (default: the whole
+    vector) and dJ/deps for every segment, computed on the device by
+    sg4_cheby_field_gradient (forward with checkpoints, adjoint sweep with
+    recomputation, forward tangents of the segments)."""
+    ns = psi0.shape[0]
+    if prop.op_mu is None:
+        prop.mu_s(psi0[:1] * 0)          # builds the mu operator plan
+    x0 = to_rows(psi0)
+    tg = to_rows(target)
+    if blocks is None:
+        blocks = [(0, psi0.shape[-1])]
+    br = (C.c_int64 * (2 * len(blocks)))(*[int(v) for b in blocks for v in b])
+    prm, keep = _params(prop, eps, steps_per_segment, checkpoint_every)
+    mu = prop.mu_grid.contiguous()
+    J = C.c_double()
+    grad = np.zeros(len(eps))
+    _capi.check(lib().sg4_cheby_field_gradient(
+        prop.op._h, prop.op_mu._h, C.c_void_p(mu.data_ptr()), ns, C.c_void_p(x0.data_ptr()),
+        C.c_void_p(tg.data_ptr()), len(blocks), br, C.byref(prm), C.byref(J),
+        grad.ctypes.data_as(_capi.D_P)))
+    return J.value, grad

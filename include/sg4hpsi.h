@@ -146,6 +146,12 @@ int sg4_plan_set_operator(sg4_plan *p, const sg4_operator *op);
  * calcN_op of Working_tests/MPI_tests/6D_Davidson_S0/sub_system_HenonHeiles.f:14-43),
  * kinetic -1/2 sum d^2 (type 1).  Needs tables->nodes at plan creation. */
 int sg4_plan_set_henon_heiles(sg4_plan *p, double lambda);
+/* Dipole mu = sum_k q_k (the coupling of the field term, sg4_plan_set_field)
+ * evaluated on the device at every grid point of every term of every
+ * instance, in the layout of the operator grids (sum_t nq_t values, DEVICE
+ * pointer).  The first-call grid evaluation of an operator that is the sum of
+ * the coordinates; needs tables->nodes at plan creation. */
+int sg4_plan_grid_dipole(sg4_plan *p, double *mu);
 /* Type-10 operator of a triatomic in Jacobi coordinates (modes R, r, theta),
  * J = 0, evaluated on the device on every term grid (the first-call grid
  * evaluation of get_OpGrid_type10_OF_ONEDP_FOR_SG4,
@@ -189,6 +195,50 @@ int sg4_apply_rows(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, i
  * segment.  The reference propagates fields with Taylor/RK only
  * (march_noD_field, SRC/sub_propagation/sub_module_propa_march.f90:1641). */
 int sg4_plan_set_field(sg4_plan *p, const double *mu, double eps);
+
+/* ---- Chebyshev propagation under a piecewise-constant field -------------
+ * march_cheby (SRC/sub_propagation/sub_module_propa_march.f90:4145-4436,
+ * recurrence :4297-4309) on the device: Hs = (H - E0)/Esc, w_0 = psi,
+ * w_1 = -i Hs w_0, w_{k+1} = w_{k-1} - 2i Hs w_k, psi(t+dt) = exp(-i E0 dt)
+ * sum_k c_k w_k with ncoef orders (the caller passes c_0 = J_0(Esc dt),
+ * c_k = 2 J_k(Esc dt), as cof does, :5739-5760).  Segment s of nseg runs
+ * steps_per_seg steps with H = H0 + eps[s] mu (sg4_plan_set_field); H0 is
+ * restored at the end.  States: ns complex states as a DEVICE rows-layout
+ * block of nb_ba x 2 ns values (row r: the real parts of the ns states, then
+ * their imaginary parts).  The recurrence runs in element-wise kernels on the
+ * plan's stream, one H.psi launch per order; no host synchronisation per
+ * order.  The reference propagates fields with Taylor/RK only
+ * (march_noD_field, :1641): Chebyshev per constant-field segment is new. */
+typedef struct {
+  double dt;              /* time step */
+  double E0, Esc;         /* centre and half-width of the spectral range */
+  int32_t ncoef;          /* Chebyshev orders per step */
+  const double *coef;     /* host: the ncoef series coefficients */
+  int32_t nseg;           /* field segments */
+  int32_t steps_per_seg;  /* steps per segment */
+  const double *eps;      /* host: nseg field amplitudes */
+  int32_t ckpt_every;     /* gradient: segments between stored states */
+  int32_t prec;           /* precision mode of the H.psi applications */
+} sg4_cheby_params;
+/* psi <- psi(T) (needs max_nvec >= 2 ns); mu: DEVICE grid of the dipole. */
+int sg4_cheby_propagate(sg4_plan *H, const double *mu, int32_t ns, double *psi,
+                        const sg4_cheby_params *prm);
+/* J = sum_i |a_i|^2, a_i = sum over the rows [blk_rows[2i], blk_rows[2i+1])
+ * and the ns states of conj(target) psi(T) (the instances of a batch), and
+ * grad[s] = dJ/deps[s] for every segment (host array of nseg), exact for the
+ * discrete propagator: the adjoint state is propagated backwards with
+ * U^dagger and contracted with the forward tangent of each segment
+ * (dHs/deps = mu/Esc, applied by Mu, a plan of the same rows holding the
+ * potential mu and no kinetic part).  The states at the start of every
+ * ckpt_every-th segment are stored and the backward sweep recomputes the
+ * others, one checkpoint interval at a time.  psi0, target: DEVICE
+ * rows-layout blocks as above; H needs max_nvec >= 4 ns, Mu >= 2 ns.  Mu runs
+ * on H's stream during the call.  No counterpart in the reference (its only
+ * derivative machinery is the field update of sub_control.f90:907-1011). */
+int sg4_cheby_field_gradient(sg4_plan *H, sg4_plan *Mu, const double *mu, int32_t ns,
+                             const double *psi0, const double *target, int32_t nblk,
+                             const int64_t *blk_rows, const sg4_cheby_params *prm, double *J,
+                             double *grad);
 
 /* Derivative of <phi_v|H|psi_v> with respect to every grid value of the
  * potential: grad[(p)*nvec + v] = W_t w(p) (B_t phi_v)(p) (B_t psi_v)(p) for the
