@@ -132,7 +132,11 @@ def propagation_bench(E, torch, args):
     spp = max(1, args.propagation // len(eps))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    psiT = propagate_device(prop, psi0, eps, spp)
+    psiT = psi0
+    for c in range(0, len(eps), 10):     # 10 segments per call, a progress line each
+        psiT = propagate_device(prop, psiT, eps[c:c + 10], spp)
+        print(f"propagation: {(c + 10) * spp} steps, {time.perf_counter() - t1:.1f} s",
+              file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     tf = time.perf_counter() - t1
     nsteps = spp * len(eps)
