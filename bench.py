@@ -182,7 +182,7 @@ def main():
                          "Henon-Heiles instances, 200-segment field, NSTEPS Chebyshev steps of "
                          "dt = 0.1 (forward), plus the field gradient over the first "
                          "--grad-steps steps")
-    ap.add_argument("--grad-steps", type=int, default=50)
+    ap.add_argument("--grad-steps", type=int, default=25)
     ap.add_argument("--D", type=int, default=10)
     ap.add_argument("--L", type=int, default=8)
     ap.add_argument("--nvec", type=int, default=16)
