@@ -248,6 +248,11 @@ class SG4Operator:
         the kinetic part on the grid (the reference's structure)."""
         _capi.check(lib().sg4_plan_set_option(self._h, _capi.OPT_GROUPING, 1 if on else 0))
 
+    def set_deterministic(self, on=True):
+        """Order-independent (bitwise reproducible) sum over the terms: 128-bit
+        fixed-point accumulation on the split type-1 path."""
+        _capi.check(lib().sg4_plan_set_option(self._h, _capi.OPT_DETERMINISTIC, 1 if on else 0))
+
     def split_info(self):
         """(split path in use, stored entries of the kinetic matrix)"""
         sp, nnz = C.c_int32(), C.c_int64()

@@ -10,6 +10,7 @@ I_P = C.POINTER(C.c_int32)
 
 PREC_FP64, PREC_F32, PREC_F32X2 = 0, 1, 2
 OPT_GROUPING = 1
+OPT_DETERMINISTIC = 2
 BASIS_HO, BASIS_LEGENDRE = 0, 1
 
 # every symbol the header declares (checked by the CPU tests)

@@ -58,7 +58,39 @@ struct KArgs {
   int64_t slot_size;
   int nvec;
   int nitems;
+  // deterministic accumulation (split path): 128-bit fixed-point accumulator
+  // per result element (two words) and an overflow counter; nullptr: FP64 atomics
+  unsigned long long* fx = nullptr;
+  unsigned long long* fxflag = nullptr;
 };
+
+// Deterministic scatter-add: c * 2^64 as a 128-bit two's complement integer
+// (the magnitude truncated below 2^-64) added to acc[0] (low word) and acc[1]
+// (high word) with 64-bit integer atomics; the carry out of the low word is
+// the one this addition produced, so the final (high, low) pair is the exact
+// integer sum whatever the order of the additions.  |c| >= 2^63 is counted
+// in *flag instead (the result is then reported as NaN).
+__device__ __forceinline__ void fx_add(unsigned long long* acc, unsigned long long* flag, double c) {
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(c);
+  const int ex = (int)((bits >> 52) & 0x7ff);
+  if (ex == 0) return;                       // zero or below 2^-1022
+  if (ex == 0x7ff) { atomicAdd(flag, 1ULL); return; }
+  const unsigned long long m = (bits & 0xfffffffffffffULL) | (1ULL << 52);
+  const int sh = ex - 1075 + 64;             // c 2^64 = m 2^sh
+  unsigned long long lo, hi;
+  if (sh >= 75) { atomicAdd(flag, 1ULL); return; }
+  if (sh >= 64) { hi = m << (sh - 64); lo = 0; }
+  else if (sh > 0) { lo = m << sh; hi = m >> (64 - sh); }
+  else if (sh > -53) { lo = m >> (-sh); hi = 0; }
+  else return;                               // below 2^-64
+  if (bits >> 63) {                          // negate (two's complement, 128 bits)
+    lo = ~lo + 1ULL;
+    hi = ~hi + (lo == 0 ? 1ULL : 0ULL);
+  }
+  const unsigned long long old = atomicAdd(&acc[0], lo);
+  const unsigned long long h = hi + ((old + lo < old) ? 1ULL : 0ULL);
+  if (h) atomicAdd(&acc[1], h);
+}
 
 // ---------------------------------------------------------------------------
 // device helpers

@@ -56,6 +56,25 @@ __global__ void k_axpy_grid(const double* __restrict__ v0, const double* __restr
     v[i] = mu ? fma(eps, mu[i], v0[i]) : v0[i];
 }
 
+// deterministic accumulation: clear the 128-bit accumulators and the overflow
+// counter; then add their values (hi * 2^64 + lo) * 2^-64 to hpsi, or NaN if
+// some contribution overflowed the fixed-point range
+__global__ void k_fx_clear(unsigned long long* __restrict__ p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0ULL;
+}
+__global__ void k_fx_finish(const unsigned long long* __restrict__ fx,
+                            const unsigned long long* __restrict__ flag, double* __restrict__ hpsi,
+                            int64_t n) {
+  const bool bad = *flag != 0ULL;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = (double)(long long)fx[2 * i + 1] + ldexp((double)fx[2 * i], -64);
+    hpsi[i] = bad ? __builtin_nan("") : hpsi[i] + v;
+  }
+}
+
 __global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -297,6 +316,11 @@ struct sg4_plan {
   double last_flops_mfma = 0;  // MFMA path: flops the matrix cores executed in that launch
   bool grouping = true;      // register-blocked group passes for square terms
   bool no_alloc = false;     // inside sg4_apply_noalloc: use prepared state only
+  // SG4_OPT_DETERMINISTIC: term contributions summed exactly in 128-bit fixed
+  // point (order-independent); d_fx holds 2 words per result element and the
+  // overflow counter at the end
+  bool deterministic = false;
+  sg4::DevBuf d_fx;
   // split type-1 path: kinetic sparse matrix K (CSR over packed rows) and the
   // host copies of the map and matrices it is assembled from
   bool split_ok = false;
@@ -1100,6 +1124,17 @@ static LaunchCfg cfg_for(sg4_plan* P, int nvec, int prec) {
 // lists); done before any stream capture.
 static int prepare(sg4_plan* P, int nvec, int prec, const LaunchCfg& cfg, ItemPlan** ip) {
   int rc;
+  if (P->deterministic) {
+    if (!use_split(P)) {
+      set_error("deterministic accumulation is available on the split type-1 path only");
+      return SG4_ERR_UNSUPPORTED;
+    }
+    const size_t need = 8 * (2 * (size_t)P->nb_ba * P->max_nvec + 2);
+    if (P->d_fx.n < need) {
+      if (P->no_alloc) return not_prepared();
+      if ((rc = P->d_fx.alloc(need))) return rc;
+    }
+  }
   const int esize = (prec == SG4_PREC_FP64) ? 8 : 4;
   if (esize == 4 && (rc = ensure_f32(P))) return rc;
   if (prec == SG4_PREC_F32X2) {
@@ -1133,6 +1168,16 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
   a.items = ip->items.as<Item>();
   a.scratch = ip->scratch.as<double>();
   a.slot_size = ip->slot_size;
+  unsigned long long* fx = nullptr;
+  unsigned long long* fxflag = nullptr;
+  if (P->deterministic) {
+    fx = P->d_fx.as<unsigned long long>();
+    fxflag = fx + 2 * n;
+    hipLaunchKernelGGL(k_fx_clear, dim3(grid_blocks(2 * n + 1)), dim3(256), 0, P->stream, fx,
+                       2 * n + 1);
+  }
+  a.fx = fx;
+  a.fxflag = fxflag;
   P->last_items = ip->nitems + ip->nitems2;
   P->last_flops = ip->flops_main;
   P->last_flops_mfma = ip->flops_mfma;
@@ -1162,6 +1207,8 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
     m.V = P->d_V32.as<float>();
     m.psi = psi;
     m.hpsi = hpsi;
+    m.fx = fx;
+    m.fxflag = fxflag;
     m.nvec = nvec;
     for (int list = 0; list < 2; ++list) {
       const int64_t ni = list ? ip->nitems2 : ip->nitems;
@@ -1175,6 +1222,9 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
       }
       if (P->timing && list == 0) HIP_TRY(hipEventRecord(P->ev1, P->stream));
     }
+    if (fx)
+      hipLaunchKernelGGL(k_fx_finish, dim3(grid_blocks(n)), dim3(256), 0, P->stream, fx, fxflag,
+                         hpsi, n);
     HIP_TRY(hipGetLastError());
     return SG4_OK;
   }
@@ -1207,6 +1257,9 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
       }
       if (P->timing && list == 0) HIP_TRY(hipEventRecord(P->ev1, P->stream));
     }
+    if (fx)
+      hipLaunchKernelGGL(k_fx_finish, dim3(grid_blocks(n)), dim3(256), 0, P->stream, fx, fxflag,
+                         hpsi, n);
   } else if (P->type_op == 10) {
     KMats10 km{P->d_mats.as<double>(), P->d_V.as<double>(), P->d_G.as<double>(),
                P->d_Jac.as<double>(), P->d_sqRJ.as<double>()};
@@ -1923,6 +1976,11 @@ int sg4_apply_graph(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, 
 
 int sg4_plan_set_option(sg4_plan* p, int32_t option, int32_t value) {
   if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
+  if (option == SG4_OPT_DETERMINISTIC) {
+    drop_graphs(p);
+    p->deterministic = value != 0;
+    return SG4_OK;
+  }
   if (option != SG4_OPT_GROUPING) { set_error("unknown option"); return SG4_ERR_ARG; }
   p->grouping = value != 0;
   invalidate(p);

@@ -60,6 +60,8 @@ struct MArgs {
   const float* V;         // FP32 potential grids
   const double* psi;      // rows layout psi[row*nvec + v]
   double* hpsi;           // rows layout H.psi (holds the kinetic part; terms are added)
+  unsigned long long* fx;       // deterministic accumulator (or nullptr), see fx_add
+  unsigned long long* fxflag;
   int nvec;
   int nitems;
 };
@@ -171,6 +173,8 @@ struct MCtx {
   const uint16_t* matB;     // fused pass: grid->basis matrix
   const double* psi;
   double* hpsi;
+  unsigned long long* fx;
+  unsigned long long* fxflag;
   uint32_t dummy;           // byte address of a per-lane scratch word (masked-out rows)
   int nvec, v0, lgnv, RS;
   int N, K, A;
@@ -268,9 +272,11 @@ __device__ __forceinline__ void mpass(const MCtx& c) {
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         const int m = c.Ms[pb + koff[s]];
-        if (cv && kval[s] && m >= 0)
-          atomicAdd((double*)((char*)c.hpsi + ((uint32_t)m * rowb + vb)),
-                    c.w * (double)y[s >> 2][s & 3]);
+        if (cv && kval[s] && m >= 0) {
+          const uint32_t off = (uint32_t)m * rowb + vb;   // bytes of the FP64 element
+          if (c.fx) fx_add(c.fx + 2 * (off >> 3), c.fxflag, c.w * (double)y[s >> 2][s & 3]);
+          else atomicAdd((double*)((char*)c.hpsi + off), c.w * (double)y[s >> 2][s & 3]);
+        }
       }
     }
   }
@@ -295,7 +301,9 @@ __device__ __forceinline__ void term_mfma(const MArgs& a, const Item& it, float*
     if (m >= 0)
       for (int v = threadIdx.x; v < nv; v += blockDim.x) {
         const int64_t r = (int64_t)m * a.nvec + it.v0 + v;
-        atomicAdd(&a.hpsi[r], w * (double)(V[0] * (float)a.psi[r]));
+        const double val = w * (double)(V[0] * (float)a.psi[r]);
+        if (a.fx) fx_add(a.fx + 2 * r, a.fxflag, val);
+        else atomicAdd(&a.hpsi[r], val);
       }
     return;
   }
@@ -317,6 +325,8 @@ __device__ __forceinline__ void term_mfma(const MArgs& a, const Item& it, float*
   c.dummy = (uint32_t)(uintptr_t)(dummy + (threadIdx.x & 63));
   c.psi = a.psi;
   c.hpsi = a.hpsi;
+  c.fx = a.fx;
+  c.fxflag = a.fxflag;
   c.nvec = a.nvec;
   c.v0 = it.v0;
   c.lgnv = lg;

@@ -30,6 +30,8 @@ struct SCtx {
   const int32_t* Ms;     // the term's map, staged in LDS
   const double* psi;     // gather: slabs read from the packed block (group 0 only)
   double* hpsi;          // scatter: slabs added into the packed result (group 0 only)
+  unsigned long long* fx;       // deterministic accumulator (or nullptr)
+  unsigned long long* fxflag;
   double w;
   int nvec, v0;
   int gather, scatter;
@@ -84,7 +86,11 @@ __device__ __forceinline__ void split_pass(const SCtx<T>& c) {
 #pragma unroll
       for (int J = 0; J < P; ++J) {
         const int m = c.Ms[P * b + J];
-        if (m >= 0) atomicAdd(&c.hpsi[(int64_t)m * c.nvec + c.v0 + av], c.w * (double)x[J]);
+        if (m >= 0) {
+          const int64_t r = (int64_t)m * c.nvec + c.v0 + av;
+          if (c.fx) fx_add(c.fx + 2 * r, c.fxflag, c.w * (double)x[J]);
+          else atomicAdd(&c.hpsi[r], c.w * (double)x[J]);
+        }
       }
     } else {
 #pragma unroll
@@ -121,7 +127,9 @@ __device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km
     if (m >= 0)
       for (int v = threadIdx.x; v < nv; v += blockDim.x) {
         const int64_t r = (int64_t)m * a.nvec + it.v0 + v;
-        atomicAdd(&a.hpsi[r], w * (double)(V[0] * (T)a.psi[r]));
+        const double val = w * (double)(V[0] * (T)a.psi[r]);
+        if (a.fx) fx_add(a.fx + 2 * r, a.fxflag, val);
+        else atomicAdd(&a.hpsi[r], val);
       }
     return;
   }
@@ -157,6 +165,8 @@ __device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km
   c.Ms = Ms;
   c.psi = a.psi;
   c.hpsi = a.hpsi;
+  c.fx = a.fx;
+  c.fxflag = a.fxflag;
   c.w = w;
   c.nvec = a.nvec;
   c.v0 = it.v0;

@@ -283,6 +283,17 @@ int sg4_apply_graph(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, 
  * pass per mode with the kinetic on the grid, as in the reference (all paths
  * give the same operator up to rounding). */
 #define SG4_OPT_GROUPING 1
+/* SG4_OPT_DETERMINISTIC (default 0), split type-1 path: the weighted
+ * contributions of the terms are summed in a 128-bit fixed-point accumulator
+ * per result element (scale 2^64, range |partial sums| < 2^63) with 64-bit
+ * integer atomics, whose sum does not depend on the order of the additions:
+ * repeated applies and every launch configuration give bitwise identical
+ * results, and the only rounding of the sum is the truncation of each
+ * contribution below 2^-64.  A contribution beyond the range makes the result
+ * NaN.  The reference scatter-adds with OpenMP atomics (order-dependent,
+ * sub_module_basis_BtoG_GtoB_SG4.f90:1288-1289); the default here is FP64
+ * atomics.  Costs 16 bytes of device memory per result element. */
+#define SG4_OPT_DETERMINISTIC 2
 int sg4_plan_set_option(sg4_plan *p, int32_t option, int32_t value);
 /* Whether applies of this plan take the split path (split = 1) and the number
  * of stored entries of its kinetic matrix.  No counterpart in the reference. */

@@ -57,6 +57,7 @@ def lib():
                                      D_P, D_P, C.c_int64, C.c_int64, C.c_int64]
         L.orc_hpsi_absbound.argtypes = [C.c_void_p, C.c_int, D_P, D_P, D_P, D_P, D_P, C.c_int,
                                         D_P, D_P]
+        L.orc_hpsi_ld.argtypes = [C.c_void_p, D_P, D_P, C.c_int, D_P, D_P]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -203,6 +204,21 @@ def _absbound(self, psi, V, ck=None):
 
 
 SG4.absbound = _absbound
+
+
+def _hpsi_ld(self, psi, V, ck):
+    """Type-1 H.psi in extended precision (long double contractions and sums)."""
+    psi = np.asarray(psi, dtype=np.float64)
+    one = psi.ndim == 1
+    pf = np.asfortranarray(psi[:, None] if one else psi)
+    out = np.zeros((self.nb_ba, pf.shape[1]), order="F")
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    ck = np.ascontiguousarray(ck, dtype=np.float64)
+    assert lib().orc_hpsi_ld(self._h, _d(V), _d(ck), pf.shape[1], _d(pf), _d(out)) == 0
+    return out[:, 0] if one else out
+
+
+SG4.hpsi_ld = _hpsi_ld
 
 
 def num_threads():

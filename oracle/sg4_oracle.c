@@ -742,6 +742,100 @@ int orc_hpsi(const orc_sg *s, int type, const double *V, const double *ck,
   return orc_hpsi_range(s, type, V, ck, G, Jac, sqRJ, nvec, psi, hpsi, 0, -1, 1);
 }
 
+/* The type-1 block action in extended precision: the same operator (the
+ * double 1-D matrices, grids and weights of orc_sg) with every mode product,
+ * the grid operator and the scatter-sum carried in long double (64-bit
+ * mantissa).  Its rounding, ~5e-20 times the Smolyak amplification, is far
+ * below FP64's, so a comparison with it measures the other side's error
+ * alone.  Test infrastructure like the rest of this file. */
+static void mode_product_ld(const long double *in, long double *out, const int *dims, int D,
+                            int k, const double *M, int m_out) {
+  int64_t A = 1, B = 1;
+  for (int j = 0; j < k; j++) A *= dims[j];
+  for (int j = k + 1; j < D; j++) B *= dims[j];
+  int m_in = dims[k];
+  for (int64_t b = 0; b < B; b++)
+    for (int i = 0; i < m_out; i++)
+      for (int64_t a = 0; a < A; a++) {
+        long double sum = 0;
+        for (int j = 0; j < m_in; j++)
+          sum += (long double)M[i + (size_t)m_out * j] * in[a + A * (j + (int64_t)m_in * b)];
+        out[a + A * (i + (int64_t)m_out * b)] = sum;
+      }
+}
+
+int orc_hpsi_ld(const orc_sg *s, const double *V, const double *ck, int nvec, const double *psi,
+                double *hpsi) {
+  const int D = s->D;
+  const int64_t nb_ba = s->nb_ba;
+  int64_t maxq = 1;
+  for (int64_t t = 0; t < s->nterms; t++) {
+    int64_t n = s->qoff[t + 1] - s->qoff[t];
+    if (n > maxq) maxq = n;
+  }
+  int nth = 1;
+#ifdef _OPENMP
+  nth = omp_get_max_threads();
+#endif
+  long double *acc = calloc((size_t)nth * nb_ba * nvec, sizeof(long double));
+  if (!acc) return -1;
+#pragma omp parallel
+  {
+    int th = 0;
+#ifdef _OPENMP
+    th = omp_get_thread_num();
+#endif
+    long double *mine = acc + (size_t)th * nb_ba * nvec;
+    long double *w1 = malloc(sizeof(long double) * maxq), *w2 = malloc(sizeof(long double) * maxq);
+    long double *w3 = malloc(sizeof(long double) * maxq);
+    int dims[ORC_MAXD];
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t t = 0; t < s->nterms; t++) {
+      const int *lv = s->lev + (size_t)D * t;
+      const int64_t nqt = s->qoff[t + 1] - s->qoff[t], nbt = s->boff[t + 1] - s->boff[t];
+      const int *mp = s->map + s->boff[t];
+      const double *Vt = V + s->qoff[t];
+      for (int v = 0; v < nvec; v++) {
+        const double *p = psi + (size_t)nb_ba * v;
+        for (int k = 0; k < D; k++) dims[k] = s->b1[k][lv[k]].nb;
+        for (int64_t e = 0; e < nbt; e++) w1[e] = mp[e] >= 0 ? (long double)p[mp[e]] : 0.0L;
+        for (int k = 0; k < D; k++) {
+          const orc_b1 *b = &s->b1[k][lv[k]];
+          mode_product_ld(w1, w2, dims, D, k, b->d0, b->nq);
+          dims[k] = b->nq;
+          memcpy(w1, w2, sizeof(long double) * nqt);
+        }
+        for (int64_t e = 0; e < nqt; e++) w3[e] = (long double)Vt[e] * w1[e];
+        for (int k = 0; k < D; k++) {
+          if (ck[k] == 0.0) continue;
+          const orc_b1 *b = &s->b1[k][lv[k]];
+          mode_product_ld(w1, w2, dims, D, k, b->gg2, b->nq);
+          for (int64_t e = 0; e < nqt; e++) w3[e] += (long double)ck[k] * w2[e];
+        }
+        for (int k = 0; k < D; k++) {
+          const orc_b1 *b = &s->b1[k][lv[k]];
+          mode_product_ld(w3, w2, dims, D, k, b->BtW, b->nb);
+          dims[k] = b->nb;
+          int64_t n = 1;
+          for (int j = 0; j < D; j++) n *= dims[j];
+          memcpy(w3, w2, sizeof(long double) * n);
+        }
+        long double *h = mine + (size_t)nb_ba * v;
+        for (int64_t e = 0; e < nbt; e++)
+          if (mp[e] >= 0) h[mp[e]] += (long double)s->weight[t] * w3[e];
+      }
+    }
+    free(w1); free(w2); free(w3);
+  }
+  for (int64_t e = 0; e < nb_ba * nvec; e++) {
+    long double sum = 0;
+    for (int th = 0; th < nth; th++) sum += acc[(size_t)th * nb_ba * nvec + e];
+    hpsi[e] = (double)sum;
+  }
+  free(acc);
+  return 0;
+}
+
 int orc_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

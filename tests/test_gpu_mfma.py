@@ -140,3 +140,23 @@ def test_f32x2_rejected_off_the_split_path(cuda):
     op.set_grouping(False)
     with pytest.raises(E.SG4Error):
         op.apply_host(np.ones((b.nb_ba, 2)), E.PREC_F32X2)
+
+
+def test_f32x2_deterministic_bitwise(cuda):
+    """With the deterministic sum the split-precision mode is bitwise
+    repeatable, also across launch configurations (the per-term MFMA results
+    do not depend on the item order or the LDS budget)."""
+    import torch
+    D, L, nvec = 6, 6, 16
+    b = E.SG4Basis(D, L)
+    op = E.SG4Operator(b, max_nvec=nvec)
+    op.set_deterministic(True)
+    X = torch.randn(b.nb_ba, nvec, device=cuda, dtype=torch.float64,
+                    generator=torch.Generator(device=cuda).manual_seed(21))
+    Y1, Y2, Y3 = (torch.empty_like(X) for _ in range(3))
+    op.apply_rows(X, Y1, E.PREC_F32X2)
+    op.apply_rows(X, Y2, E.PREC_F32X2)
+    op.set_config(nvec, E.PREC_F32X2, nthreads=512, lds_budget=163840, grid=1024, xcd=3)
+    op.apply_rows(X, Y3, E.PREC_F32X2)
+    torch.cuda.synchronize()
+    assert torch.equal(Y1, Y2) and torch.equal(Y1, Y3)

@@ -258,3 +258,20 @@ def test_type10_jacobi_3d_ground_level_bound():
     H8 = s8.hpsi(np.eye(s8.nb_ba), V8, None, type_op=10, G=G8.reshape(-1), Jac=np.sin(t8),
                  sqRJ=np.ones_like(t8))
     assert abs(np.min(np.linalg.eigvals(H8).real) - e0) < 1e-5 * e0
+
+
+@pytest.mark.parametrize("D,L", [(3, 4), (6, 3), (6, 6)])
+def test_extended_precision_action_matches(D, L):
+    """orc_hpsi_ld is the same discrete operator as orc_hpsi evaluated in long
+    double: the two agree within eps times the conditioning bound of the
+    Smolyak sum, and the extended one is linear to far below FP64 rounding."""
+    o = O.SG4(D, L)
+    V = o.grid_hh(0.111803)
+    ck = -0.5 * np.ones(D)
+    rng = np.random.default_rng(D * 10 + L)
+    psi = rng.standard_normal((o.nb_ba, 3))
+    a, b = o.hpsi(psi, V, ck), o.hpsi_ld(psi, V, ck)
+    bound = o.absbound(psi, V, ck)
+    assert np.max(np.abs(a - b)) <= 2 * np.finfo(float).eps * np.max(bound)
+    lin = o.hpsi_ld(2 * psi[:, :1] - 3 * psi[:, 1:2], V, ck)[:, 0]
+    assert np.max(np.abs(lin - (2 * b[:, 0] - 3 * b[:, 1]))) < 1e-14 * np.max(np.abs(lin))
