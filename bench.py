@@ -46,7 +46,7 @@ UNIT = {"fp64": "FP64 VALU (= FP64 MFMA peak)", "f32": "FP32 VALU",
 # threads, LDS budget, persistent grid, item order (3: classes of terms with
 # equal levels in modes 3..D-1 on one XCD; 2: modes 2..D-1)
 PINNED = {(10, 8, 16): {"fp64": (512, 163840, 2048, 3), "f32": (768, 163840, 2048, 2),
-                        "f32x2": (512, 163840, 1024, 3)}}
+                        "f32x2": (1024, 163840, 512, 2)}}
 PREC = {"fp64": 0, "f32": 1, "f32x2": 2}
 DTYPE = {"fp64": "f64", "f32": "f32", "f32x2": "bf16x3-split/f32-acc"}
 KERNEL = {"fp64": "k_hpsi_t1s_f64", "f32": "k_hpsi_t1s_f32", "f32x2": "k_hpsi_mfma"}

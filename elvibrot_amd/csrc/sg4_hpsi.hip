@@ -1217,7 +1217,7 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
         m.nitems = (int)ni;
         m.items = list ? ip->items2.as<Item>() : ip->items.as<Item>();
         const unsigned gr = (unsigned)((cfg.grid > 0 && cfg.grid < ni) ? cfg.grid : ni);
-        hipLaunchKernelGGL(k_hpsi_mfma, dim3(gr), dim3(std::min(cfg.nthreads, NT)),
+        hipLaunchKernelGGL(k_hpsi_mfma, dim3(gr), dim3(std::min(cfg.nthreads, NT_MFMA)),
                            list ? ip->lds2 : ip->lds, P->stream, m);
       }
       if (P->timing && list == 0) HIP_TRY(hipEventRecord(P->ev1, P->stream));
@@ -1868,8 +1868,8 @@ int sg4_plan_autotune(sg4_plan* p, int32_t nvec, int32_t prec, float* best_ms) {
       cus = prop.multiProcessorCount;
   }
   const int threads3[] = {256, 512, NT32};
-  const std::vector<int> threads(threads3,
-                                 threads3 + ((use_split(p) && prec == SG4_PREC_F32) ? 3 : 2));
+  std::vector<int> threads(threads3, threads3 + ((use_split(p) && prec == SG4_PREC_F32) ? 3 : 2));
+  if (prec == SG4_PREC_F32X2) threads.push_back(NT_MFMA);
   const int budgets[] = {32 * 1024, 48 * 1024, 64 * 1024, 80 * 1024, 112 * 1024, 160 * 1024};
   const int grids[] = {0, 4, 8};   // 0: one per item; k: k workgroups per CU, persistent
   for (int nt : threads)
@@ -1915,10 +1915,12 @@ int sg4_plan_config(const sg4_plan* p, int32_t nvec, int32_t prec, int32_t* cfg4
 int sg4_plan_set_config(sg4_plan* p, int32_t nvec, int32_t prec, const int32_t* cfg4) {
   if (!p || !cfg4) { set_error("null argument"); return SG4_ERR_ARG; }
   const bool t768 = cfg4[0] == NT32 && use_split(p) && prec == SG4_PREC_F32;
-  if ((cfg4[0] != 256 && cfg4[0] != 512 && !t768) || cfg4[1] < 1024 || cfg4[1] > LDS_MAX ||
+  const bool t1024 = cfg4[0] == NT_MFMA && prec == SG4_PREC_F32X2;
+  if ((cfg4[0] != 256 && cfg4[0] != 512 && !t768 && !t1024) || cfg4[1] < 1024 || cfg4[1] > LDS_MAX ||
       cfg4[2] < 0 || cfg4[3] < 0 || cfg4[3] > 8) {
-    set_error("launch configuration out of range (threads 256/512, or 768 for the FP32 split "
-              "path; 1 KiB <= LDS budget <= 160 KiB, grid >= 0, 0 <= xcd <= 8)");
+    set_error("launch configuration out of range (threads 256/512, 768 for the FP32 split "
+              "path, 1024 for the split-precision mode; 1 KiB <= LDS budget <= 160 KiB, "
+              "grid >= 0, 0 <= xcd <= 8)");
     return SG4_ERR_ARG;
   }
   LaunchCfg c;

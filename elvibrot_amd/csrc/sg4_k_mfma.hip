@@ -3,6 +3,6 @@
 
 namespace sg4 {
 
-__global__ __launch_bounds__(NT) void k_hpsi_mfma(MArgs a) { mfma_items(a); }
+__global__ __launch_bounds__(NT_MFMA) void k_hpsi_mfma(MArgs a) { mfma_items(a); }
 
 }  // namespace sg4

@@ -39,6 +39,7 @@
 namespace sg4 {
 
 constexpr int MG_MAX = 16;          // groups per term
+constexpr int NT_MFMA = 1024;       // largest workgroup of the MFMA term kernel
 constexpr int MAT_STRIDE = 3072;    // bf16 entries per group matrix (3 limbs, 32 x 32)
 
 // A term of the MFMA path.
