@@ -124,11 +124,13 @@ class SG4Operator:
       * ``("jacobi_triatomic", (mu_R, mu_r, D, a, Re, re, c))`` -- the type-10
         triatomic model, V, G, Jac evaluated on the device grids;
       * a numpy array ``V`` of length ``basis.sum_nq`` with ``ck`` (type 1) or
-        with ``G``, ``Jac``, ``sqRJ`` (type 10).
+        with ``G``, ``Jac``, ``sqRJ`` (type 10);
+      * ``("type1", der, F)`` -- the general type-1 operator
+        sum_e F[e](q) D_e with derivative terms der[e] = (i, j), -1 = none.
     """
 
     def __init__(self, basis, max_nvec=16, potential=("henon_heiles", 0.111803), ck=None,
-                 stream=None, type_op=1, G=None, Jac=None, sqRJ=None):
+                 stream=None, type_op=1, G=None, Jac=None, sqRJ=None, der=None, F=None):
         self.basis = basis
         self.max_nvec = max_nvec
         self.type_op = type_op
@@ -139,6 +141,14 @@ class SG4Operator:
             _capi.check(lib().sg4_plan_set_stream(h, C.c_void_p(stream)))
         if isinstance(potential, tuple) and potential[0] == "henon_heiles":
             _capi.check(lib().sg4_plan_set_henon_heiles(h, float(potential[1])))
+            return
+        if isinstance(potential, tuple) and potential[0] == "type1":
+            # general type 1: der = [(i, j), ...] (-1 = none), F: (nder, sum_nq) grids
+            d = np.ascontiguousarray(np.asarray(potential[1], dtype=np.int32).reshape(-1, 2))
+            Fa = np.ascontiguousarray(potential[2], dtype=np.float64).reshape(len(d), basis.sum_nq)
+            _capi.check(lib().sg4_plan_set_operator_type1(
+                h, len(d), d.ctypes.data_as(C.POINTER(C.c_int32)), Fa.ctypes.data_as(_capi.D_P)))
+            self.type_op = 11
             return
         if isinstance(potential, tuple) and potential[0] == "jacobi_triatomic":
             prm = (C.c_double * 7)(*[float(x) for x in potential[1]])

@@ -23,7 +23,7 @@ EXPORTS = [
     "sg4_plan_kernel_times", "sg4_grad_V", "sg4_plan_set_field", "sg4_plan_autotune", "sg4_plan_config", "sg4_plan_set_config",
     "sg4_apply_graph", "sg4_plan_set_option", "sg4_plan_split_info", "sg4_plan_mfma_stats",
     "sg4_plan_destroy", "sg4_apply_noalloc", "sg4_cheby_propagate", "sg4_cheby_field_gradient",
-    "sg4_plan_grid_dipole",
+    "sg4_plan_grid_dipole", "sg4_plan_set_operator_type1",
 ]
 
 
@@ -98,6 +98,7 @@ def lib():
         L.sg4_plan_mfma_stats.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.sg4_plan_destroy.argtypes = [vp]
         L.sg4_plan_grid_dipole.argtypes = [vp, vp]
+        L.sg4_plan_set_operator_type1.argtypes = [vp, i32, C.POINTER(i32), D_P]
         L.sg4_cheby_propagate.argtypes = [vp, vp, i32, vp, C.POINTER(ChebyParams)]
         L.sg4_cheby_field_gradient.argtypes = [vp, vp, vp, i32, vp, vp, i32,
                                                C.POINTER(C.c_int64), C.POINTER(ChebyParams),

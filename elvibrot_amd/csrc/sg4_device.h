@@ -487,6 +487,16 @@ struct KMats10 {
   const double* sqRJ;
 };
 
+// general type-1 operator: derivative terms (two mode indices each, -1 =
+// none) and their grid coefficients F[e * sum_nq + point]
+struct KMats1G {
+  const double* mats;
+  const double* F;
+  const int32_t* der;
+  int nder;
+  int64_t sum_nq;
+};
+
 struct KGrad {
   const double* mats;
   const double* qw;        // 1-D quadrature weights
@@ -506,5 +516,6 @@ __global__ void k_hpsi_t1_f64(KArgs a, KMats<double> km);
 __global__ void k_hpsi_t1_f32(KArgs a, KMats<float> km);
 __global__ void k_hpsi_t10_f64(KArgs a, KMats10 km);
 __global__ void k_grad_v(KArgs a, KGrad kg);
+__global__ void k_hpsi_t1g_f64(KArgs a, KMats1G km);
 
 }  // namespace sg4

@@ -321,6 +321,9 @@ struct sg4_plan {
   // overflow counter at the end
   bool deterministic = false;
   sg4::DevBuf d_fx;
+  // general type-1 operator (type_op 11): derivative terms and their grids
+  int nder = 0;
+  sg4::DevBuf d_der, d_F;
   // split type-1 path: kinetic sparse matrix K (CSR over packed rows) and the
   // host copies of the map and matrices it is assembled from
   bool split_ok = false;
@@ -1104,8 +1107,8 @@ static int check_apply(sg4_plan* P, int nvec, int prec) {
     set_error("unknown precision mode");
     return SG4_ERR_ARG;
   }
-  if (P->type_op == 10 && prec != SG4_PREC_FP64) {
-    set_error("the type-10 operator is available in FP64 only");
+  if ((P->type_op == 10 || P->type_op == 11) && prec != SG4_PREC_FP64) {
+    set_error("the type-10 and general type-1 operators are available in FP64 only");
     return SG4_ERR_UNSUPPORTED;
   }
   return SG4_OK;
@@ -1141,6 +1144,7 @@ static int prepare(sg4_plan* P, int nvec, int prec, const LaunchCfg& cfg, ItemPl
     if ((rc = build_mfma(P))) return rc;
     return get_items_mfma(P, nvec, cfg.lds_budget, cfg.xcd, ip);
   }
+  if (P->type_op == 11) return get_items(P, nvec, esize, ip, 4, LDS_MAX, 0, false);
   const bool t10 = P->type_op == 10;
   return get_items(P, nvec, esize, ip, 0, t10 ? LDS_SMALL : cfg.lds_budget, t10 ? 0 : cfg.xcd,
                    use_split(P));
@@ -1260,6 +1264,10 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
     if (fx)
       hipLaunchKernelGGL(k_fx_finish, dim3(grid_blocks(n)), dim3(256), 0, P->stream, fx, fxflag,
                          hpsi, n);
+  } else if (P->type_op == 11) {
+    KMats1G km{P->d_mats.as<double>(), P->d_F.as<double>(), P->d_der.as<int32_t>(), P->nder,
+               P->sum_nq};
+    hipLaunchKernelGGL(k_hpsi_t1g_f64, dim3(grid), dim3(256), ip->lds, P->stream, a, km);
   } else if (P->type_op == 10) {
     KMats10 km{P->d_mats.as<double>(), P->d_V.as<double>(), P->d_G.as<double>(),
                P->d_Jac.as<double>(), P->d_sqRJ.as<double>()};
@@ -1402,6 +1410,8 @@ int sg4_plan_create_batch(int32_t ninst, const sg4_tables* tabs, int32_t max_nve
     (void)hipFuncSetAttribute((const void*)k_grad_v,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
     (void)hipFuncSetAttribute((const void*)k_hpsi_mfma,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
+    (void)hipFuncSetAttribute((const void*)k_hpsi_t1g_f64,
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
   });
   std::unique_ptr<sg4_plan> P(new sg4_plan);
@@ -1585,6 +1595,28 @@ int sg4_plan_set_operator(sg4_plan* p, const sg4_operator* op) {
   }
   p->type_op = op->type_op;
   HIP_TRY(hipMemcpy(p->d_V.p, op->V, sizeof(double) * p->sum_nq, hipMemcpyHostToDevice));
+  (void)p->d_V0.alloc(0);
+  return build_terms(p);
+}
+
+int sg4_plan_set_operator_type1(sg4_plan* p, int32_t nder, const int32_t* der, const double* F) {
+  if (!p || !der || !F || nder < 1) { set_error("null argument"); return SG4_ERR_ARG; }
+  if (p->inst.size() != 1) {
+    set_error("the general type-1 operator needs a single-instance plan");
+    return SG4_ERR_UNSUPPORTED;
+  }
+  if (p->D > MAXACT) { set_error("more than 16 modes"); return SG4_ERR_UNSUPPORTED; }
+  for (int e = 0; e < 2 * nder; ++e)
+    if (der[e] < -1 || der[e] >= p->D) { set_error("derivative mode index out of range"); return SG4_ERR_ARG; }
+  invalidate(p);
+  int rc;
+  if ((rc = p->d_der.alloc(sizeof(int32_t) * 2 * nder))) return rc;
+  if ((rc = p->d_F.alloc(sizeof(double) * (size_t)nder * p->sum_nq))) return rc;
+  HIP_TRY(hipMemcpy(p->d_der.p, der, sizeof(int32_t) * 2 * nder, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(p->d_F.p, F, sizeof(double) * (size_t)nder * p->sum_nq, hipMemcpyHostToDevice));
+  p->nder = nder;
+  p->type_op = 11;
+  p->h_ck.assign(p->nck, 0.0);
   (void)p->d_V0.alloc(0);
   return build_terms(p);
 }
@@ -1805,6 +1837,10 @@ int sg4_plan_split_info(const sg4_plan* p, int32_t* split, int64_t* kin_nnz) {
 int sg4_plan_set_field(sg4_plan* p, const double* mu, double eps) {
   if (!p) { set_error("null plan"); return SG4_ERR_ARG; }
   if (p->type_op == 0) { set_error("no operator set on the plan"); return SG4_ERR_STATE; }
+  if (p->type_op == 11) {
+    set_error("field terms of the general type-1 operator go into its identity term");
+    return SG4_ERR_UNSUPPORTED;
+  }
   int rc;
   if (!p->d_V0.p) {
     if ((rc = p->d_V0.alloc(sizeof(double) * p->sum_nq))) return rc;
@@ -1823,6 +1859,10 @@ int sg4_grad_V(sg4_plan* p, int32_t nvec, const double* phi, const double* psi, 
   if (p->type_op == 0) { set_error("no operator set on the plan"); return SG4_ERR_STATE; }
   if (!p->have_qw) { set_error("plan was created without quadrature weights"); return SG4_ERR_STATE; }
   if (nvec < 1 || nvec > p->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
+  if (p->type_op == 11) {
+    set_error("grid-value gradients are available for the type-1 (V, c_k) and type-10 operators");
+    return SG4_ERR_UNSUPPORTED;
+  }
   ItemPlan* ip = nullptr;
   int rc = get_items(p, nvec, 8, &ip, 4);
   if (rc) return rc;

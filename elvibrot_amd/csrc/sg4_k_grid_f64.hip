@@ -204,3 +204,121 @@ __global__ __launch_bounds__(NT) void k_grad_v(KArgs a, KGrad kg) {
 }
 
 }  // namespace sg4
+
+namespace sg4 {
+
+// ---------------------------------------------------------------------------
+// General type-1 operator, FP64 (sub_TabOpPsi_OF_ONEDP_FOR_SGtype4 case 1,
+// SRC/sub_Operator/sub_OpPsi_SG4.f90:1488-1544):  on the grid of a term
+//   H psi = sum_e F_e(q) D_e psi,
+// D_e from the two mode indices of term e (-1 = none): the identity, dnRGG
+// d1 along i, d2 along i, or d1 along j then along i (the 1-D matrices of
+// DerivOp_TO_RDP_OF_SmolaykRep).  Every mode takes part.  LDS: four buffers
+// of nq_t x nv -- the two transform buffers (the grid psi X and a free one W),
+// the accumulator Y and a temporary T for the mixed derivatives.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void term_t1g(const KArgs& a, const KMats1G& km, const Item& it,
+                                         double* lds) {
+  const TermDev* Td = &a.terms[it.term];
+  const int lg = it.lgnv, nv = 1 << lg;
+  const int nact = Td->nact;
+  const int64_t SZ = (int64_t)Td->nq * nv;
+  double* b0 = lds;
+  double* b1 = b0 + SZ;
+  double* Y = b1 + SZ;
+  double* T = Y + SZ;
+  const int nt = blockDim.x;
+  PassCtx<double> c;
+  c.map = a.map + Td->boff; c.psi = a.psi; c.hpsi = a.hpsi; c.nvec = a.nvec; c.v0 = it.v0;
+  c.wgt = Td->wgt; c.V = nullptr; c.vshift = 0.0; c.lgnv = lg; c.coef = 1.0;
+  // basis -> grid
+  int A = 1, Bafter = Td->nb;
+  double* cur = nullptr;
+  double* nxt = b0;
+  for (int p = 0; p < nact; ++p) {
+    const MatDev* md = &a.md[Td->mat[p]];
+    const int nq = md->nq, nb = md->nb;
+    Bafter /= nb;
+    c.MR = km.mats + md->B;
+    c.in = cur; c.out = nxt;
+    c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
+    c.nfib = A * nv * Bafter;
+    if (p == 0) dispatch_pass<double, true, IN_GATHER, OUT_STORE>(c, nb, nq);
+    else dispatch_pass<double, true, IN_LDS, OUT_STORE>(c, nb, nq);
+    __syncthreads();
+    A *= nq;
+    cur = nxt;
+    nxt = (nxt == b0) ? b1 : b0;
+  }
+  double* X = cur;
+  double* W = nxt;
+  for (int64_t e = threadIdx.x; e < SZ; e += nt) Y[e] = 0.0;
+  // one derivative along mode p of the grid tensor: in -> out
+  auto deriv = [&](int p, int64_t moff, const double* in, double* out) {
+    int Ab = 1, Aa = Td->nq;
+    for (int q = 0; q < p; ++q) Ab *= a.md[Td->mat[q]].nq;
+    const MatDev* md = &a.md[Td->mat[p]];
+    Aa /= Ab * md->nq;
+    c.MR = km.mats + moff;
+    c.in = in; c.out = out;
+    c.A = Ab; c.AV = Ab * nv; c.invAV = 1.0f / (float)c.AV;
+    c.nfib = Ab * nv * Aa;
+    dispatch_pass<double, true, IN_LDS, OUT_STORE>(c, md->nq, md->nq);
+  };
+  __syncthreads();
+  for (int e = 0; e < km.nder; ++e) {
+    int i = km.der[2 * e], j = km.der[2 * e + 1];
+    if (i < 0 && j >= 0) { i = j; j = -1; }
+    const double* src = X;
+    if (i >= 0) {
+      const MatDev* mi = &a.md[Td->mat[i]];
+      if (j < 0 || j == i) {
+        deriv(i, j < 0 ? mi->D1 : mi->D2, X, W);
+        src = W;
+      } else {
+        deriv(j, a.md[Td->mat[j]].D1, X, W);
+        __syncthreads();
+        deriv(i, mi->D1, W, T);
+        src = T;
+      }
+      __syncthreads();
+    }
+    const double* Fe = km.F + (int64_t)e * km.sum_nq + Td->qoff;
+    for (int64_t idx = threadIdx.x; idx < SZ; idx += nt) Y[idx] += Fe[idx >> lg] * src[idx];
+    __syncthreads();
+  }
+  // grid -> basis, last pass scatters
+  cur = Y;
+  nxt = W;
+  A = 1;
+  int Aq = Td->nq;
+  for (int p = 0; p < nact; ++p) {
+    const MatDev* md = &a.md[Td->mat[p]];
+    const int nq = md->nq, nb = md->nb;
+    Aq /= nq;
+    c.MR = km.mats + md->BtW;
+    c.in = cur; c.out = nxt;
+    c.A = A; c.AV = A * nv; c.invAV = 1.0f / (float)c.AV;
+    c.nfib = A * nv * Aq;
+    if (p == nact - 1) {
+      dispatch_pass<double, true, IN_LDS, OUT_SCATTER>(c, nq, nb);
+    } else {
+      dispatch_pass<double, true, IN_LDS, OUT_STORE>(c, nq, nb);
+      __syncthreads();
+    }
+    A *= nb;
+    double* t = cur; cur = nxt; nxt = t;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_hpsi_t1g_f64(KArgs a, KMats1G km) {
+  extern __shared__ double lds[];
+  for (int w = blockIdx.x; w < a.nitems; w += gridDim.x) {
+    if (w != (int)blockIdx.x) __syncthreads();
+    const Item it = a.items[w];
+    if (it.term < 0) continue;
+    term_t1g(a, km, it, lds);
+  }
+}
+
+}  // namespace sg4

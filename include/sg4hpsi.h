@@ -141,6 +141,19 @@ int sg4_plan_create_batch(int32_t ninst, const sg4_tables *tables, int32_t max_n
 int sg4_plan_set_stream(sg4_plan *p, void *hip_stream);
 /* copy the operator grids to the device (host pointers) */
 int sg4_plan_set_operator(sg4_plan *p, const sg4_operator *op);
+/* General type-1 operator (Type_HamilOp = 1 with grid-dependent
+ * coefficients: get_OpGrid_type1_OF_ONEDP_FOR_SG4 and case 1 of
+ * sub_TabOpPsi_OF_ONEDP_FOR_SGtype4, SRC/sub_Operator/sub_OpPsi_SG4.f90
+ * :1488-1544, :2818-3082): on the grid of every term
+ *   H psi = sum_e F_e(q) D_e psi,   e = 0 .. nder-1,
+ * with D_e given by der[2e], der[2e+1] (0-based modes, -1 = none; the
+ * reference's derive_termQdyn(:,e) minus one): (-1,-1) the identity (the
+ * potential term), (i,-1) dnRGG%d1 along mode i, (i,i) dnRGG%d2, (i,j) i != j
+ * d1 along j and along i (DerivOp_TO_RDP_OF_SmolaykRep).  F: host array of
+ * nder x sum_t nq_t grid values, F[e * sum_nq + point] (the reference's
+ * GridOp(:,1,1,e) for one electronic state).  Single-instance plans, FP64
+ * applies; sg4_plan_set_field and sg4_grad_V do not apply to it. */
+int sg4_plan_set_operator_type1(sg4_plan *p, int32_t nder, const int32_t *der, const double *F);
 /* evaluate the Henon-Heiles grid V on the device (first-call grid evaluation
  * of get_OpGrid_type1_OF_ONEDP_FOR_SG4, sub_OpPsi_SG4.f90:2978-3058, with
  * calcN_op of Working_tests/MPI_tests/6D_Davidson_S0/sub_system_HenonHeiles.f:14-43),

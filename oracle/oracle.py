@@ -58,6 +58,7 @@ def lib():
         L.orc_hpsi_absbound.argtypes = [C.c_void_p, C.c_int, D_P, D_P, D_P, D_P, D_P, C.c_int,
                                         D_P, D_P]
         L.orc_hpsi_ld.argtypes = [C.c_void_p, D_P, D_P, C.c_int, D_P, D_P]
+        L.orc_hpsi_t1g.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), D_P, C.c_int, D_P, D_P]
         L.orc_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -219,6 +220,23 @@ def _hpsi_ld(self, psi, V, ck):
 
 
 SG4.hpsi_ld = _hpsi_ld
+
+
+def _hpsi_t1g(self, psi, der, F):
+    """General type-1 H.psi: sum_e F[e] D_e psi with der = [(i, j), ...]
+    (0-based modes, -1 = none) and F of shape (nder, sum_nq)."""
+    psi = np.asarray(psi, dtype=np.float64)
+    one = psi.ndim == 1
+    pf = np.asfortranarray(psi[:, None] if one else psi)
+    out = np.zeros((self.nb_ba, pf.shape[1]), order="F")
+    d = np.ascontiguousarray(np.asarray(der, dtype=np.int32).reshape(-1, 2))
+    Fa = np.ascontiguousarray(F, dtype=np.float64).reshape(len(d), self.sum_nq)
+    assert lib().orc_hpsi_t1g(self._h, len(d), d.ctypes.data_as(C.POINTER(C.c_int)), _d(Fa),
+                              pf.shape[1], _d(pf), _d(out)) == 0
+    return out[:, 0] if one else out
+
+
+SG4.hpsi_t1g = _hpsi_t1g
 
 
 def num_threads():

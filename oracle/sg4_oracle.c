@@ -742,6 +742,82 @@ int orc_hpsi(const orc_sg *s, int type, const double *V, const double *ck,
   return orc_hpsi_range(s, type, V, ck, G, Jac, sqRJ, nvec, psi, hpsi, 0, -1, 1);
 }
 
+/* General type-1 operator (Type_HamilOp = 1 with grid-dependent
+ * coefficients; sub_TabOpPsi_OF_ONEDP_FOR_SGtype4 case 1,
+ * SRC/sub_Operator/sub_OpPsi_SG4.f90:1488-1544): on the grid of a term,
+ *   H psi = sum_e F_e(q) D_e psi,
+ * D_e = derivative term e given by two mode indices (derive_termQdyn(:,e),
+ * 0-based here, -1 = none): (-1,-1) the identity, (i,-1) dnRGG%d1 along
+ * mode i, (i,i) dnRGG%d2 along mode i, (i,j) i != j dnRGG%d1 along j then
+ * along i (DerivOp_TO_RDP_OF_SmolaykRep, sub_module_basis_BtoG_GtoB_SG4.f90
+ * :2670-2775, one 1-D matrix per mode from Get3_MatdnRGG).  F: nder x sum_nq,
+ * F[e * sum_nq + point].  Block action as orc_hpsi (fixed-order sum). */
+static void deriv_apply(const orc_sg *s, const int *lv, const int *dims, int i, int j,
+                        const double *in, double *out, double *tmp) {
+  const int D = s->D;
+  if (i < 0 && j >= 0) { i = j; j = -1; }
+  int64_t n = 1;
+  for (int k = 0; k < D; k++) n *= dims[k];
+  if (i < 0) { memcpy(out, in, sizeof(double) * n); return; }
+  const orc_b1 *bi = &s->b1[i][lv[i]];
+  if (j < 0) { mode_product(in, out, dims, D, i, bi->gg1, bi->nq); return; }
+  if (i == j) { mode_product(in, out, dims, D, i, bi->gg2, bi->nq); return; }
+  const orc_b1 *bj = &s->b1[j][lv[j]];
+  mode_product(in, tmp, dims, D, j, bj->gg1, bj->nq);
+  mode_product(tmp, out, dims, D, i, bi->gg1, bi->nq);
+}
+
+int orc_hpsi_t1g(const orc_sg *s, int nder, const int *der, const double *F, int nvec,
+                 const double *psi, double *hpsi) {
+  const int D = s->D;
+  const int64_t nb_ba = s->nb_ba, sum_nq = s->qoff[s->nterms];
+  int64_t maxq = 1;
+  for (int64_t t = 0; t < s->nterms; t++) {
+    int64_t n = s->qoff[t + 1] - s->qoff[t];
+    if (n > maxq) maxq = n;
+  }
+  memset(hpsi, 0, sizeof(double) * nb_ba * nvec);
+  double *w1 = malloc(sizeof(double) * maxq), *w2 = malloc(sizeof(double) * maxq);
+  double *w3 = malloc(sizeof(double) * maxq), *w4 = malloc(sizeof(double) * maxq);
+  double *w5 = malloc(sizeof(double) * maxq);
+  int dims[ORC_MAXD];
+  for (int64_t t = 0; t < s->nterms; t++) {
+    const int *lv = s->lev + (size_t)D * t;
+    const int64_t nqt = s->qoff[t + 1] - s->qoff[t], nbt = s->boff[t + 1] - s->boff[t];
+    const int *mp = s->map + s->boff[t];
+    for (int v = 0; v < nvec; v++) {
+      const double *p = psi + (size_t)nb_ba * v;
+      for (int k = 0; k < D; k++) dims[k] = s->b1[k][lv[k]].nb;
+      for (int64_t e = 0; e < nbt; e++) w1[e] = mp[e] >= 0 ? p[mp[e]] : 0.0;
+      for (int k = 0; k < D; k++) {           /* basis -> grid */
+        const orc_b1 *b = &s->b1[k][lv[k]];
+        mode_product(w1, w2, dims, D, k, b->d0, b->nq);
+        dims[k] = b->nq;
+        memcpy(w1, w2, sizeof(double) * nqt);
+      }
+      for (int64_t e = 0; e < nqt; e++) w3[e] = 0.0;
+      for (int e = 0; e < nder; e++) {        /* sum_e F_e D_e psi on the grid */
+        const double *Fe = F + (size_t)e * sum_nq + s->qoff[t];
+        deriv_apply(s, lv, dims, der[2 * e], der[2 * e + 1], w1, w4, w5);
+        for (int64_t q = 0; q < nqt; q++) w3[q] += Fe[q] * w4[q];
+      }
+      for (int k = 0; k < D; k++) {           /* grid -> basis */
+        const orc_b1 *b = &s->b1[k][lv[k]];
+        mode_product(w3, w2, dims, D, k, b->BtW, b->nb);
+        dims[k] = b->nb;
+        int64_t n = 1;
+        for (int j = 0; j < D; j++) n *= dims[j];
+        memcpy(w3, w2, sizeof(double) * n);
+      }
+      double *h = hpsi + (size_t)nb_ba * v;
+      for (int64_t e = 0; e < nbt; e++)
+        if (mp[e] >= 0) h[mp[e]] += s->weight[t] * w3[e];
+    }
+  }
+  free(w1); free(w2); free(w3); free(w4); free(w5);
+  return 0;
+}
+
 /* The type-1 block action in extended precision: the same operator (the
  * double 1-D matrices, grids and weights of orc_sg) with every mode product,
  * the grid operator and the scatter-sum carried in long double (64-bit

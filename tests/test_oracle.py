@@ -275,3 +275,68 @@ def test_extended_precision_action_matches(D, L):
     assert np.max(np.abs(a - b)) <= 2 * np.finfo(float).eps * np.max(bound)
     lin = o.hpsi_ld(2 * psi[:, :1] - 3 * psi[:, 1:2], V, ck)[:, 0]
     assert np.max(np.abs(lin - (2 * b[:, 0] - 3 * b[:, 1]))) < 1e-14 * np.max(np.abs(lin))
+
+
+def _grid_points(o):
+    return np.concatenate([o.term_grid(t) for t in range(o.nterms)])
+
+
+def test_general_type1_reduces_to_constant_diagonal():
+    """orc_hpsi_t1g with the identity term (F = V) and d2 terms of constant
+    coefficient equals the type-1 action with constant c_k."""
+    D, L = 4, 4
+    o = O.SG4(D, L)
+    V = o.grid_hh(0.111803)
+    ck = np.array([-0.5, -0.3, -0.7, -0.5])
+    psi = np.random.default_rng(1).standard_normal((o.nb_ba, 3))
+    der = [(-1, -1)] + [(k, k) for k in range(D)]
+    F = np.vstack([V] + [ck[k] * np.ones(o.sum_nq) for k in range(D)])
+    a, b = o.hpsi(psi, V, ck), o.hpsi_t1g(psi, der, F)
+    assert np.max(np.abs(a - b)) < 1e-14 * np.max(np.abs(a))
+
+
+def test_general_type1_sheared_oscillator_spectrum():
+    """Mixed second derivatives: H = -1/2 sum_ij G_ij d_i d_j + V with
+    V = 1/2 |Omega A q|^2 and G = A^-1 A^-T is a harmonic oscillator in the
+    sheared coordinates y = A q, spectrum sum_k w_k (n_k + 1/2) exactly; the
+    2D L=12 Smolyak operator in q reproduces its 5 lowest levels to 1e-11
+    (swapping a mode index of the mixed terms or dropping them fails)."""
+    o = O.SG4(2, 12)
+    q = _grid_points(o)
+    A = np.array([[1.0, 0.3], [0.0, 1.0]])
+    w = np.array([1.0, 1.3])
+    V = 0.5 * np.sum((w ** 2) * (q @ A.T) ** 2, axis=1)
+    Gm = np.linalg.inv(A) @ np.linalg.inv(A).T
+    der = [(-1, -1), (0, 0), (1, 1), (0, 1), (1, 0)]
+    one = np.ones(len(V))
+    F = np.vstack([V, -0.5 * Gm[0, 0] * one, -0.5 * Gm[1, 1] * one, -0.5 * Gm[0, 1] * one,
+                   -0.5 * Gm[1, 0] * one])
+    ev = np.sort(np.linalg.eigvals(o.hpsi_t1g(np.eye(o.nb_ba), der, F)).real)[:5]
+    ex = np.sort([w[0] * (a + 0.5) + w[1] * (b + 0.5) for a in range(6) for b in range(6)])[:5]
+    assert np.max(np.abs(ev - ex)) < 1e-11
+
+
+def test_general_type1_equals_type10_form_when_converged():
+    """Grid-dependent coefficients and first derivatives: the operator
+    -1/2 sum_ij d_i G_ij(q) d_j + V in its type-10 form (d1 G d1) and in its
+    type-1 form (-1/2 G_ij d_i d_j - 1/2 (sum_i d_i G_ij) d_j, the derivative
+    of G taken analytically) converge to the same levels; at 2D L=12 the 4
+    lowest agree to 1e-10 (dropping the first-derivative terms moves them by
+    ~1e-3)."""
+    o = O.SG4(2, 12)
+    q = _grid_points(o)
+    n = len(q)
+    g00, g11, g01 = 1 + 0.1 * q[:, 1] ** 2, 1 + 0.1 * q[:, 0] ** 2, 0.05 * q[:, 0] * q[:, 1]
+    V = 0.5 * (q[:, 0] ** 2 + 1.2 * q[:, 1] ** 2)
+    G = np.zeros((n, 2, 2))
+    G[:, 0, 0], G[:, 1, 1], G[:, 0, 1], G[:, 1, 0] = g00, g11, g01, g01
+    H10 = o.hpsi(np.eye(o.nb_ba), V, None, type_op=10, G=G.reshape(n, 4), Jac=np.ones(n),
+                 sqRJ=np.ones(n))
+    der = [(-1, -1), (0, 0), (1, 1), (0, 1), (1, 0), (0, -1), (1, -1)]
+    F = np.vstack([V, -0.5 * g00, -0.5 * g11, -0.5 * g01, -0.5 * g01, -0.025 * q[:, 0],
+                   -0.025 * q[:, 1]])
+    e10 = np.sort(np.linalg.eigvals(H10).real)[:4]
+    e1 = np.sort(np.linalg.eigvals(o.hpsi_t1g(np.eye(o.nb_ba), der, F)).real)[:4]
+    assert np.max(np.abs(e1 - e10)) < 1e-10
+    e1x = np.sort(np.linalg.eigvals(o.hpsi_t1g(np.eye(o.nb_ba), der[:5], F[:5])).real)[:4]
+    assert np.max(np.abs(e1x - e10)) > 1e-4
