@@ -271,16 +271,25 @@ def write_restart(path, T, psis):
 
 def read_restart(path, nb_wp, nb_tot):
     """ReadWP_restart (sub_module_propagation.f90:429-495): returns (T, psis)
-    with psis (nb_wp, nb_tot) complex.  A file whose first record does not
-    hold ``T nb_WP nb_tot`` means no restart, as in the reference (a warning
-    there, T0 = 0): (0.0, None).  Sizes different from nb_wp, nb_tot raise, as
-    the reference stops."""
+    with psis (nb_wp, nb_tot) complex.  A file that does not start with
+    ``T nb_WP nb_tot`` (one list-directed read, over one or more records)
+    means no restart, as in the reference (a warning there, T0 = 0):
+    (0.0, None).  Sizes different from nb_wp, nb_tot raise, as the reference
+    stops."""
     with open(path) as f:
         lines = f.read().splitlines()
+    # the header is one list-directed read of three values: like any such
+    # read it continues onto the next records until it has them
+    head, pos = [], 0
     try:
-        head = _ld_values(lines[0]) if lines else []
-        T, nwp_f, nb_f = float(head[0].real), int(head[1].real), int(head[2].real)
-        if head[1].imag or head[2].imag or head[1].real != nwp_f or head[2].real != nb_f:
+        while len(head) < 3 and pos < len(lines):
+            head.extend(_ld_values(lines[pos]))
+            pos += 1
+        # a complex constant is not a value a read of reals and integers accepts
+        if any(isinstance(v, complex) for v in head[:3]):
+            raise ValueError
+        T, nwp_f, nb_f = float(head[0]), int(head[1]), int(head[2])
+        if head[1] != nwp_f or head[2] != nb_f:
             raise ValueError
     except (ValueError, IndexError, AttributeError):
         return 0.0, None
@@ -290,7 +299,6 @@ def read_restart(path, nb_wp, nb_tot):
     psis = np.zeros((nb_wp, nb_tot), dtype=np.complex128)
     # every packet is one list-directed read: it starts on a new record and
     # reads nb_tot complex constants across as many records as they take
-    pos = 1
     for j in range(nb_wp):
         got = []
         while len(got) < nb_tot:

@@ -159,6 +159,21 @@ def test_restart_reader_fortran_text_missing_time_and_size_checks(tmp_path):
         psi_io.read_restart(str(p), 2, 4)
     p.write_text(" (1.0,0.0) (2.0,0.0)\n")    # no T record: no restart, T0 = 0
     assert psi_io.read_restart(str(p), 1, 2) == (0.0, None)
+    p.write_text(" (1.0,0.0) (2.0,0.0)\n (3.0,0.0)\n")
+    assert psi_io.read_restart(str(p), 1, 2) == (0.0, None)
+
+
+def test_restart_header_over_several_records(tmp_path):
+    """``read(nioWP,*) T, nb_WP, nb_tot`` is one list-directed read: the three
+    values may sit on separate records, and the packets start on the record
+    after the last of them."""
+    p = tmp_path / "r"
+    p.write_text(" 0.25D0\n 1,\n 2\n (1.0,2.0) (3.0,-4.0)\n")
+    T, got = psi_io.read_restart(str(p), 1, 2)
+    assert T == 0.25 and np.array_equal(got, np.array([[1 + 2j, 3 - 4j]]))
+    p.write_text(" 0.25 1\n 2 \n (5.0,6.0)\n (7.0,8.0)\n")
+    T, got = psi_io.read_restart(str(p), 1, 2)
+    assert T == 0.25 and np.array_equal(got, np.array([[5 + 6j, 7 + 8j]]))
 
 
 def test_propagation_writes_restart_every_step_and_resumes(tmp_path):
