@@ -11,6 +11,8 @@ I_P = C.POINTER(C.c_int32)
 PREC_FP64, PREC_F32, PREC_F32X2 = 0, 1, 2
 OPT_GROUPING = 1
 OPT_DETERMINISTIC = 2
+# return codes (include/sg4hpsi.h)
+OK, ERR_ARG, ERR_HIP, ERR_OOM, ERR_UNSUPPORTED, ERR_STATE = 0, -1, -2, -3, -4, -5
 BASIS_HO, BASIS_LEGENDRE = 0, 1
 
 # every symbol the header declares (checked by the CPU tests)

@@ -333,6 +333,7 @@ int sg4_cheby_propagate(sg4_plan* H, const double* mu, int32_t ns, double* psi,
   int rc;
   if ((rc = check_params(H, ns, prm, 2))) return rc;
   if (!psi) { set_error("null argument"); return SG4_ERR_ARG; }
+  if ((rc = device_block(H, psi, "psi")) || (mu && (rc = device_block(H, mu, "mu")))) return rc;
   Prop P{};
   P.H = H; P.Mu = nullptr; P.mu = mu; P.ns = ns; P.rows = plan_rows(H);
   P.n = P.rows * 2 * ns; P.st = (hipStream_t)plan_stream(H); P.p = prm;
@@ -361,6 +362,9 @@ int sg4_cheby_field_gradient(sg4_plan* H, sg4_plan* Mu, const double* mu, int32_
     set_error("the mu operator must have the rows of H and hold 2 ns vectors");
     return SG4_ERR_ARG;
   }
+  if ((rc = device_block(H, psi0, "psi0")) || (rc = device_block(H, target, "target")) ||
+      (rc = device_block(H, mu, "mu")) || (rc = device_block(Mu, mu, "mu")))
+    return rc;
   const int64_t rows = plan_rows(H);
   for (int i = 0; i < nblk; ++i)
     if (blk_rows[2 * i] < 0 || blk_rows[2 * i + 1] > rows || blk_rows[2 * i] >= blk_rows[2 * i + 1]) {

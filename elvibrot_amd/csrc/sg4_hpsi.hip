@@ -287,6 +287,7 @@ struct sg4_plan {
   int64_t nterms = 0, nb_ba = 0, sum_nb = 0, sum_nq = 0;
   int max_nvec = 0;
   int type_op = 0;
+  int device = 0;       // the GPU current when the plan was created
   hipStream_t stream = nullptr;
   std::vector<int32_t> h_levels, h_nq, h_nb;
   std::vector<int64_t> h_qoff, h_boff;
@@ -1288,12 +1289,54 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
 
 using namespace sg4;
 
+namespace sg4 {
+// The kernels add into their outputs with hardware floating-point atomics
+// compiled with -munsafe-fp-atomics; those are correct on coarse-grained
+// device memory only.  On host (pinned), managed or fine-grained memory the
+// adds may be lost, so every block handed over by pointer must be ordinary
+// device memory (hipMalloc, hipMallocAsync, a torch CUDA tensor) on the plan's
+// GPU.  hipPointerGetAttributes looks the pointer up in the runtime's table
+// of allocations; it allocates nothing and enqueues nothing.
+int device_block(const sg4_plan* p, const void* ptr, const char* name) {
+  hipPointerAttribute_t at{};
+  const hipError_t e = hipPointerGetAttributes(&at, ptr);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();   // the lookup failure is this call's answer, not a sticky error
+    set_error(std::string(name) + " is not memory allocated by HIP: device pointers are required");
+    return SG4_ERR_ARG;
+  }
+  if (at.type != hipMemoryTypeDevice || at.isManaged) {
+    set_error(std::string(name) + " is host or managed memory: device pointers are required");
+    return SG4_ERR_ARG;
+  }
+  // hipDeviceMallocFinegrained (1), hipMallocSignalMemory (2) and
+  // hipDeviceMallocUncached (3) all set one of the two low flag bits
+  if (at.allocationFlags & 3u) {
+    set_error(std::string(name) + " is fine-grained or uncached device memory: "
+              "floating-point atomics need coarse-grained memory");
+    return SG4_ERR_ARG;
+  }
+  if (at.device != p->device) {
+    set_error(std::string(name) + " is on GPU " + std::to_string(at.device) +
+              ", the plan on GPU " + std::to_string(p->device));
+    return SG4_ERR_ARG;
+  }
+  return SG4_OK;
+}
+}  // namespace sg4
+
 extern "C" {
 
 const char* sg4_last_error(void) { return g_last_error.c_str(); }
 
 // The kernels read psi while they write or scatter-add into hpsi: the two
 // nb_ba x nvec blocks must be distinct memory.
+
+static int device_blocks(const sg4_plan* p, const double* psi, const double* hpsi) {
+  int rc = sg4::device_block(p, psi, "psi");
+  return rc ? rc : sg4::device_block(p, hpsi, "hpsi");
+}
+
 static bool blocks_overlap(const sg4_plan* p, int32_t nvec, const double* psi, const double* hpsi) {
   if (nvec < 1) return false;
   const uintptr_t a = (uintptr_t)psi, b = (uintptr_t)hpsi;
@@ -1415,6 +1458,7 @@ int sg4_plan_create_batch(int32_t ninst, const sg4_tables* tabs, int32_t max_nve
                               hipFuncAttributeMaxDynamicSharedMemorySize, LDS_MAX);
   });
   std::unique_ptr<sg4_plan> P(new sg4_plan);
+  HIP_TRY(hipGetDevice(&P->device));
   P->max_nvec = max_nvec;
   bool nodes = true, qws = true;
   int64_t off = 0, noff = 0;
@@ -1710,12 +1754,14 @@ int sg4_plan_set_jacobi_triatomic(sg4_plan* p, const double* prm) {
 
 int sg4_apply_rows(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_t prec) {
   if (!p || !psi || !hpsi) { set_error("null argument"); return SG4_ERR_ARG; }
+  if (int rc0 = device_blocks(p, psi, hpsi)) return rc0;
   if (blocks_overlap(p, nvec, psi, hpsi)) return SG4_ERR_ARG;
   return apply_rows(p, nvec, psi, hpsi, prec);
 }
 
 int sg4_apply(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_t prec) {
   if (!p || !psi || !hpsi) { set_error("null argument"); return SG4_ERR_ARG; }
+  if (int rc0 = device_blocks(p, psi, hpsi)) return rc0;
   if (nvec < 1 || nvec > p->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
   if (blocks_overlap(p, nvec, psi, hpsi)) return SG4_ERR_ARG;
   const int64_t n = p->nb_ba * nvec;
@@ -1763,6 +1809,7 @@ extern "C" {
 
 int sg4_apply_noalloc(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_t prec) {
   if (!p || !psi || !hpsi) { set_error("null argument"); return SG4_ERR_ARG; }
+  if (int rc0 = device_blocks(p, psi, hpsi)) return rc0;
   if (nvec < 1 || nvec > p->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
   if (blocks_overlap(p, nvec, psi, hpsi)) return SG4_ERR_ARG;
   const size_t n = (size_t)p->nb_ba * nvec;
@@ -1842,6 +1889,7 @@ int sg4_plan_set_field(sg4_plan* p, const double* mu, double eps) {
     return SG4_ERR_UNSUPPORTED;
   }
   int rc;
+  if (mu && (rc = sg4::device_block(p, mu, "mu"))) return rc;
   if (!p->d_V0.p) {
     if ((rc = p->d_V0.alloc(sizeof(double) * p->sum_nq))) return rc;
     HIP_TRY(hipMemcpyAsync(p->d_V0.p, p->d_V.p, sizeof(double) * p->sum_nq,
@@ -1863,8 +1911,10 @@ int sg4_grad_V(sg4_plan* p, int32_t nvec, const double* phi, const double* psi, 
     set_error("grid-value gradients are available for the type-1 (V, c_k) and type-10 operators");
     return SG4_ERR_UNSUPPORTED;
   }
+  int rc;
+  if ((rc = device_blocks(p, phi, psi)) || (rc = sg4::device_block(p, grad, "grad"))) return rc;
   ItemPlan* ip = nullptr;
-  int rc = get_items(p, nvec, 8, &ip, 4);
+  rc = get_items(p, nvec, 8, &ip, 4);
   if (rc) return rc;
   KArgs a;
   a.terms = p->d_terms.as<TermDev>();
@@ -1973,6 +2023,7 @@ int sg4_plan_set_config(sg4_plan* p, int32_t nvec, int32_t prec, const int32_t* 
 int sg4_apply_graph(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_t prec) {
   if (!p || !psi || !hpsi) { set_error("null argument"); return SG4_ERR_ARG; }
   int rc;
+  if ((rc = device_blocks(p, psi, hpsi))) return rc;
   if ((rc = check_apply(p, nvec, prec))) return rc;
   if (blocks_overlap(p, nvec, psi, hpsi)) return SG4_ERR_ARG;
   hipStream_t user = p->stream;

@@ -50,6 +50,9 @@ void set_error(const std::string& msg);
 void* plan_stream(const sg4_plan* p);
 int64_t plan_rows(const sg4_plan* p);
 int plan_max_nvec(const sg4_plan* p);
+// SG4_OK when ptr is coarse-grained device memory on the plan's GPU, else
+// SG4_ERR_ARG with a message naming the argument
+int device_block(const sg4_plan* p, const void* ptr, const char* name);
 
 }  // namespace sg4
 

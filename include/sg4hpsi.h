@@ -179,7 +179,15 @@ int sg4_plan_set_jacobi_triatomic(sg4_plan *p, const double *prm);
  * pointers to nb_ba x nvec column-major blocks; hpsi is overwritten.  The two
  * blocks must not overlap (distinct objects, as Psi and OpPsi are in the
  * reference): overlapping blocks are rejected with SG4_ERR_ARG.  The same holds
- * for sg4_apply_rows and sg4_apply_graph. */
+ * for sg4_apply_rows and sg4_apply_graph.
+ * Every DEVICE pointer of this interface (here and in sg4_apply_noalloc,
+ * sg4_apply_rows, sg4_apply_graph, sg4_plan_set_field, sg4_grad_V and the
+ * sg4_cheby_* calls) must be ordinary coarse-grained device memory of the GPU
+ * the plan was created on (hipMalloc, hipMallocAsync, a torch CUDA tensor): the
+ * kernels sum into their outputs with hardware floating-point atomics, which
+ * are exact on that memory only.  Host, pinned, managed, fine-grained or
+ * uncached memory, and memory of another GPU, is rejected with SG4_ERR_ARG
+ * before anything is enqueued. */
 int sg4_apply(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);
 /* Allocation-free variant of sg4_apply, for callers that cannot allocate
  * device memory (code under HIP graph capture, serving loops, harnesses that
