@@ -226,6 +226,31 @@ def test_type10_jacobi_parity(cuda, L, nvec):
     assert rel(got, ref) < TOL
 
 
+@pytest.mark.parametrize("D,L,nvec", [(2, 8, 3), (3, 5, 4), (5, 4, 2)])
+def test_type10_nonsymmetric_G_general_grids_parity(cuda, D, L, nvec):
+    """Type 10 with host grids in which nothing is special: G a random
+    non-symmetric D x D matrix per point, Jac and sqRhoOVERJac random positive
+    (the oracle's type-10 form is pinned for this case against its
+    product-rule expansion)."""
+    print(f"case D={D} L={L} nvec={nvec}", flush=True)
+    b = E.SG4Basis(D, L)
+    o = O.SG4(D, L)
+    rng = np.random.default_rng(70 + D)
+    n = o.sum_nq
+    V = rng.standard_normal(n)
+    G = rng.standard_normal(n * D * D)
+    Jac = 0.5 + rng.random(n)
+    sqRJ = 0.5 + rng.random(n)
+    op = E.SG4Operator(b, 8, potential=V, type_op=10, G=G, Jac=Jac, sqRJ=sqRJ)
+    psi = rng.standard_normal((b.nb_ba, nvec))
+    ref = o.hpsi(psi, V, None, type_op=10, G=G, Jac=Jac, sqRJ=sqRJ)
+    assert rel(op.apply_host(psi), ref) < TOL
+    # the transposed G is a different operator, as the device computes it too
+    Gt = G.reshape(n, D, D).transpose(0, 2, 1).ravel()
+    op_t = E.SG4Operator(b, 8, potential=V, type_op=10, G=Gt, Jac=Jac, sqRJ=sqRJ)
+    assert rel(op_t.apply_host(psi), ref) > 1e-3
+
+
 def test_autotuned_config_keeps_parity(cuda):
     import torch
     D, L, nvec = 6, 6, 16
