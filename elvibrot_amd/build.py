@@ -62,6 +62,7 @@ def build(verbose=False):
 
 
 FORTRAN = os.path.join(HERE, "fortran")
+ROCM_LIB = "/opt/rocm/lib"
 FBUILD = os.path.join(HERE, "_build")
 
 
@@ -71,13 +72,16 @@ def build_fortran(verbose=False):
     os.makedirs(FBUILD, exist_ok=True)
     fc = "/opt/rocm/bin/amdflang" if os.path.exists("/opt/rocm/bin/amdflang") else "amdflang"
     mod_o = os.path.join(FBUILD, "mod_sg4_gpu.o")
-    drv_o = os.path.join(FBUILD, "test_sg4_fortran.o")
-    exe = os.path.join(FBUILD, "test_sg4_fortran")
-    cmds = [
-        [fc, "-O2", "-c", os.path.join(FORTRAN, "mod_sg4_gpu.f90"), "-J", FBUILD, "-o", mod_o],
-        [fc, "-O2", "-c", os.path.join(FORTRAN, "test_sg4_fortran.f90"), "-J", FBUILD, "-o", drv_o],
-        [fc, drv_o, mod_o, "-L" + HERE, "-lsg4hpsi", "-Wl,-rpath,$ORIGIN/..", "-o", exe],
-    ]
+    cmds = [[fc, "-O2", "-c", os.path.join(FORTRAN, "mod_sg4_gpu.f90"), "-J", FBUILD, "-o", mod_o]]
+    exes = []
+    for drv in ("test_sg4_fortran", "test_sg4_reflayout"):
+        drv_o = os.path.join(FBUILD, drv + ".o")
+        exe = os.path.join(FBUILD, drv)
+        cmds += [[fc, "-O2", "-c", os.path.join(FORTRAN, drv + ".f90"), "-J", FBUILD, "-o", drv_o],
+                 # the reference-layout driver allocates device blocks itself (hipMalloc)
+                 [fc, drv_o, mod_o, "-L" + HERE, "-lsg4hpsi", "-L" + ROCM_LIB, "-lamdhip64",
+                  "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + ROCM_LIB, "-o", exe]]
+        exes.append(exe)
     for c in cmds:
         if verbose:
             print(" ".join(c), flush=True)
@@ -85,7 +89,7 @@ def build_fortran(verbose=False):
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError("amdflang failed")
-    return exe
+    return exes
 
 
 if __name__ == "__main__":

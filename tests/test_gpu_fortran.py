@@ -26,3 +26,82 @@ def test_fortran_host_block_apply(cuda, tmp_path):
     o = O.SG4(6, 3)
     ref = o.hpsi(psi, o.grid_hh(0.111803), -0.5 * np.ones(6))
     assert np.max(np.abs(hpsi - ref)) / np.max(np.abs(ref)) < 1e-12
+
+
+REFLAYOUT = os.path.join(ROOT, "elvibrot_amd", "_build", "test_sg4_reflayout")
+
+
+def _write_reference_layout(path, o, type_op, grids, psi, max_nvec):
+    """The oracle's SG4 tables in the reference's conventions: Tab_nDval(D,nb_SG)
+    0-based levels, WeightSG, tab_iB_OF_SRep_TO_iB 1-based with 0 = absent,
+    and per (k, l) dnRGB%d0 (nq,nb), dnRBGwrho%d0 (nb,nq) = d0^T wrho,
+    dnRGG%d1, dnRGG%d2 (nq,nq), x, wrho -- all column-major."""
+    f64 = lambda a: np.asfortranarray(a, dtype=np.float64).tobytes(order="F")
+    i32 = lambda a: np.asarray(a, dtype=np.int32).tobytes(order="F")
+    D, L = o.D, o.L
+    nb_ba, nvec = psi.shape
+    hdr = [D, L, o.nterms, nb_ba, o.sum_nb, o.sum_nq, nvec, type_op, max_nvec]
+    tab_iB = o.map.astype(np.int64) + 1                      # -1 (absent) -> 0
+    assert tab_iB.min() >= 0 and tab_iB.max() <= nb_ba
+    parts = [i32(hdr), i32(o.lev.T), f64(o.weight), i32(tab_iB)]
+    for k in range(D):
+        for l in range(L + 1):
+            b = o.basis1d(k, l)
+            d0bwrho = b["d0"].T * b["w"][None, :]
+            parts += [i32([b["nq"], b["nb"]]), f64(b["d0"]), f64(d0bwrho), f64(b["gg1"]),
+                      f64(b["gg2"]), f64(b["x"]), f64(b["w"])]
+    parts += [f64(g) for g in grids]
+    parts.append(f64(psi))
+    path.write_bytes(b"".join(parts))
+
+
+CASES = {
+    # HO modes of unequal kinetic coefficients, LB < L (absent map entries)
+    "type1_HO_LB": dict(o=lambda: O.SG4(4, 5, LB=3), type_op=1),
+    # two Morse-like HO modes and a Legendre mode
+    "type1_mixed": dict(o=lambda: O.SG4(3, 6, kind=[0, 0, 1], nq_A=2, nb_A=2), type_op=1),
+    # type 10 with a random non-symmetric G, Jac and sqRhoOVERJac
+    "type10_general": dict(o=lambda: O.SG4(3, 4), type_op=10),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_fortran_reference_layout_tables_host_and_device(cuda, tmp_path, case):
+    """Tables handed over in the reference's layout (built by the oracle's own
+    set-up, not by the library) through sg4_gpu_init_BasisnD; H.psi through
+    host arrays and through device blocks the Fortran program hipMallocs,
+    5 vectors with max_nvec = 2; both match the oracle at 1e-12, and a host
+    array given to the device entry is refused with SG4_ERR_ARG."""
+    assert os.path.exists(REFLAYOUT), "Fortran driver not built (run __graft_entry__.build())"
+    c = CASES[case]
+    o = c["o"]()
+    rng = np.random.default_rng(sum(map(ord, case)))
+    n = o.sum_nq
+    V = rng.standard_normal(n)
+    if c["type_op"] == 1:
+        ck = -rng.uniform(0.2, 1.0, o.D)
+        grids = [V, ck]
+        ref_of = lambda p: o.hpsi(p, V, ck)
+    else:
+        G = rng.standard_normal(n * o.D * o.D)
+        Jac, sq = 0.5 + rng.random(n), 0.5 + rng.random(n)
+        grids = [V, G, Jac, sq]
+        ref_of = lambda p: o.hpsi(p, V, None, type_op=10, G=G, Jac=Jac, sqRJ=sq)
+    psi = rng.standard_normal((o.nb_ba, 5))
+    fin, fout = tmp_path / "tables.bin", tmp_path / "hpsi.bin"
+    _write_reference_layout(fin, o, c["type_op"], grids, psi, max_nvec=2)
+    print(f"case {case} nb_ba={o.nb_ba}", flush=True)
+    r = subprocess.run([REFLAYOUT, str(fin), str(fout)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = fout.read_bytes()
+    m = o.nb_ba * 5
+    out = np.frombuffer(raw[:16 * m], dtype=np.float64)
+    host = out[:m].reshape(5, o.nb_ba).T
+    dev = out[m:].reshape(5, o.nb_ba).T
+    refused = int(np.frombuffer(raw[16 * m:], dtype=np.int32)[0])
+    ref = ref_of(psi)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(host - ref)) / scale < 1e-12
+    assert np.max(np.abs(dev - ref)) / scale < 1e-12
+    assert refused == -1                                       # SG4_ERR_ARG
