@@ -17,7 +17,10 @@ Formats (SRC/sub_WP/sub_module_psi_io.f90):
 * version 1, formatted: ``&headerFile`` namelist (Version_File, nb_psi,
   nb_tot), the index list (one ``i_1 .. i_D i_h i_e`` line per packed row,
   Write_list_nDindBasis :1370-1477), then per vector ``T`` and the
-  ``nb_tot`` coefficients, list-directed (Write_Psi_nDBasis case 1).
+  ``nb_tot`` coefficients, list-directed (Write_Psi_nDBasis case 1).  The
+  reference's reader for this case (Read_psi_nDBasis, :797-812) reads the
+  vector without the leading T its writer puts there; the files here follow
+  the writer, and the reader here takes T first.
 * version 1, unformatted (sequential records with 4-byte length markers):
   ``'X'``, nb_psi, nb_tot, 1, ``'X'``, nb_tot index records, one record
   ``T, coefficients`` per vector.
@@ -28,7 +31,9 @@ Formats (SRC/sub_WP/sub_module_psi_io.f90):
 
 Complex values are written as list-directed complex constants ``(re,im)``;
 the readers take the forms list-directed input allows (blanks or commas,
-``D`` exponents, repeat counts ``r*c``, values over several records).
+``D`` exponents, repeat counts ``r*c``, values over several records, a
+complex constant split over two records, namelist items ``NAME= value``),
+with each READ starting on a new record as in Fortran.
 Only one electronic state and no extra index (``i_h = i_e = 1``) occur for
 the SG4 bases here.
 """
@@ -74,6 +79,46 @@ def _ld_values(text):
             v = float(tok.replace("D", "E").replace("d", "e"))
         out.extend([v] * (int(rpt) if rpt else 1))
     return out
+
+
+def _ld_read(lines, pos, n, what="value"):
+    """One list-directed READ of n values from the records ``lines[pos:]``:
+    it starts on a new record, takes values from successive records until it
+    has n, and the next READ starts on the record after the last one used
+    (values left on that record are skipped).  Returns (values, next_pos)."""
+    got, carry = [], ""
+    while len(got) < n:
+        if pos >= len(lines):
+            raise ValueError(f"end of file inside a list-directed read of {n} {what}s")
+        text = carry + " " + lines[pos]
+        pos += 1
+        # a complex constant may continue on the next record, before or after
+        # its comma (list-directed output splits long ones that way)
+        m = _LD_OPEN.search(text)
+        carry, text = (text[m.start():], text[:m.start()]) if m else ("", text)
+        got.extend(_ld_values(text))
+    return got[:n], pos
+
+
+_LD_OPEN = re.compile(r"(?:\d+\*)?\([^()]*$")
+_NML_ITEM = re.compile(r"([A-Za-z_]\w*)\s*=\s*([^,/\s]+)")
+
+
+def _namelist(lines, pos):
+    """The items ``NAME = value`` of a namelist group starting at record pos
+    (``&group``, items separated by commas or blanks, over one or more
+    records, closed by ``/``); names upper-cased.  Returns (dict, next_pos)."""
+    text = []
+    while True:
+        if pos >= len(lines):
+            raise ValueError("end of file inside the &headerFile namelist")
+        text.append(lines[pos])
+        pos += 1
+        if "/" in text[-1]:
+            break
+    body = " ".join(text).split("/", 1)[0]
+    body = re.sub(r"^\s*&\w+", "", body)
+    return {k.upper(): v for k, v in _NML_ITEM.findall(body)}, pos
 
 
 def _as_complex(vals, n):
@@ -218,31 +263,29 @@ def read_psi(path, basis, cplx=False, formatted=True):
     # version 1: &headerFile namelist, possibly over several lines
     if not lines[0].lstrip().startswith("&"):
         raise ValueError("unrecognised wave-packet file header")
-    text, pos = [], 0
-    while True:
-        text.append(lines[pos])
-        pos += 1
-        if "/" in text[-1]:
-            break
-    hdr = " ".join(text).upper().replace(",", " ")
-    kv = dict(tok.split("=") for tok in hdr.split() if "=" in tok)
+    kv, pos = _namelist(lines, 0)
     n, nb = int(kv["NB_PSI"]), int(kv["NB_TOT"])
     info.update(version=int(kv.get("VERSION_FILE", 1)), nb_psi=n, nb_tot=nb)
-    lst = [tuple(int(x) for x in lines[pos + r].split()[:D]) for r in range(nb)]
-    pos += nb
+    # one READ per packed row: ind(1:D), i_h, i_e
+    lst = []
+    for r in range(nb):
+        v, pos = _ld_read(lines, pos, D + 2, "index")
+        lst.append(tuple(int(x.real) for x in v[:D]))
     target = np.array([rows.get(t, -1) for t in lst])
-    vals = _ld_values("\n".join(lines[pos:]))
     psi = np.zeros((nbr, n), dtype=dt)
     ok = target >= 0
-    q = 0
     for j in range(n):
-        info["T"].append(float(vals[q].real))
-        q += 1
+        # one READ per vector: T, vec(1:nb_tot)
+        vals, pos2 = _ld_read(lines, pos, nb + 1, "coefficient")
+        if cplx and not all(isinstance(x, complex) for x in vals[1:]):
+            # (re, im) pairs of reals (files written before the complex constants)
+            vals, pos2 = _ld_read(lines, pos, 2 * nb + 1, "coefficient")
+        pos = pos2
+        info["T"].append(float(vals[0].real))
         if cplx:
-            v, used = _as_complex(vals[q:], nb)
+            v, _ = _as_complex(vals[1:], nb)
         else:
-            v, used = np.array([x.real for x in vals[q:q + nb]]), nb
-        q += used
+            v = np.array([x.real for x in vals[1:]])
         psi[target[ok], j] = v[ok]
         info["skipped"] += int((~ok).sum())
     return psi, info
@@ -300,13 +343,11 @@ def read_restart(path, nb_wp, nb_tot):
     # every packet is one list-directed read: it starts on a new record and
     # reads nb_tot complex constants across as many records as they take
     for j in range(nb_wp):
-        got = []
-        while len(got) < nb_tot:
-            if pos >= len(lines):
-                raise ValueError(f"restart file ends inside packet {j + 1}")
-            got.extend(_ld_values(lines[pos]))
-            pos += 1
-        if not all(isinstance(v, complex) for v in got[:nb_tot]):
+        try:
+            got, pos = _ld_read(lines, pos, nb_tot, "coefficient")
+        except ValueError:
+            raise ValueError(f"restart file ends inside packet {j + 1}") from None
+        if not all(isinstance(v, complex) for v in got):
             raise ValueError(f"packet {j + 1}: complex constants (re,im) expected")
-        psis[j] = got[:nb_tot]
+        psis[j] = got
     return T, psis

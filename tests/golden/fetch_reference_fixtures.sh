@@ -10,3 +10,11 @@ mkdir -p "$OUT/hermquadra"
 for n in $(seq 1 40); do cp "$REF/Internal_data/HermQuadra/herm$n.txt" "$OUT/hermquadra/"; done
 cp "$REF/Working_tests/MPI_tests/6D_Davidson_S0/benchmark" "$OUT/hh6d_L3_benchmark.txt"
 cp "$REF/Working_tests/MPI_tests/21D_Davidson_S0/benchmark" "$OUT/hh21d_L2_benchmark.txt"
+# initial wave packets (version-0 lines, 'end wp0' closed) held inline in the
+# exa_hcn test inputs
+H=$REF/Working_tests/exa_hcn-test
+sed -n '71,74p' "$H/dat_2DSG_type4"            > "$OUT/psi_v0_2d_dat_2DSG_type4.txt"
+sed -n '80,87p' "$H/dat_Davidson"              > "$OUT/psi_v0_3d_dat_Davidson.txt"
+sed -n '67,70p' "$H/dat_Davidson_1D_WithGrid"  > "$OUT/psi_v0_1d_dat_Davidson_1D_WithGrid.txt"
+sed -n '68,77p' "$H/dat_Davidson_WithGrid"     > "$OUT/psi_v0_3d_dat_Davidson_WithGrid.txt"
+sed -n '65,68p' "$H/dat_Davidson_AutoContract" > "$OUT/psi_v0_2d_dat_Davidson_AutoContract.txt"

@@ -4,7 +4,11 @@ The packed-row index tuples written to the files are checked against the
 oracle's packed basis (lex order of the reference's nDindB); the line format
 against the version-0 wave-packet lines that the reference's own test input
 Working_tests/exa_hcn-test/dat_2DSG_type4:71-74 holds (fixture
-tests/golden/psi_v0_2d_dat_2DSG_type4.txt); the rest by round trips."""
+tests/golden/psi_v0_2d_dat_2DSG_type4.txt) and the other inline packets of
+the exa_hcn inputs; version 1 and the restart file both ways through a
+Fortran runtime (tests/fortran_io/read_wp_files.f90 built with amdflang:
+files written here read with the reference writer's statements, files that
+runtime writes read here); the rest by round trips."""
 import os
 import struct
 
@@ -208,3 +212,145 @@ def test_propagation_writes_restart_every_step_and_resumes(tmp_path):
     rest, _, _ = propagate_field(Stub(), torch.tensor(mid), eps[2:], 2, restart_path=str(p), t0=T)
     assert np.array_equal(rest.numpy(), full.numpy())
     assert abs(psi_io.read_restart(str(p), 2, 5)[0] - 0.6) < 1e-15
+
+
+@pytest.mark.parametrize("name,D,L,tuples", [
+    ("psi_v0_1d_dat_Davidson_1D_WithGrid.txt", 1, 4, [(1,), (2,)]),
+    ("psi_v0_2d_dat_Davidson_AutoContract.txt", 2, 4, [(1, 1), (1, 2)]),
+    ("psi_v0_3d_dat_Davidson.txt", 3, 4, [(1, 1, 1), (2, 1, 1), (1, 2, 1), (1, 1, 2)]),
+    ("psi_v0_3d_dat_Davidson_WithGrid.txt", 3, 4,
+     [(1, 1, 1), (4, 1, 1), (6, 1, 1), (1, 1, 2), (1, 2, 1)]),
+])
+def test_reference_inline_wave_packets(tmp_path, name, D, L, tuples):
+    """The initial packets of the reference's exa_hcn inputs (version-0 lines
+    'i_1 .. i_D i_h i_e a', closed by 'end wp0', one with a leading blank):
+    each is the unit vector on the packed row of its 1-based index tuple."""
+    b = E.SG4Basis(D, L)
+    body = open(os.path.join(HERE, "golden", name)).read()
+    p = tmp_path / "wp.txt"
+    p.write_text(f" {len(tuples)}\n" + body)
+    psi, info = psi_io.read_psi(str(p), b)
+    assert info["nb_psi"] == len(tuples) and info["skipped"] == 0
+    idx = psi_io.packed_indices(b)
+    want = np.zeros((b.nb_ba, len(tuples)))
+    for j, tup in enumerate(tuples):
+        want[int(np.nonzero((idx == np.array(tup) - 1).all(1))[0][0]), j] = 1.0
+    assert np.array_equal(psi, want)
+
+
+@pytest.fixture(scope="module")
+def fortran_reader(tmp_path_factory):
+    """tests/fortran_io/read_wp_files.f90 built with amdflang."""
+    exe = tmp_path_factory.mktemp("fio") / "read_wp_files"
+    src = os.path.join(HERE, "fortran_io", "read_wp_files.f90")
+    r = __import__("subprocess").run(["/opt/rocm/bin/amdflang", "-O1", src, "-o", str(exe)],
+                                     capture_output=True, text=True, cwd=str(exe.parent))
+    assert r.returncode == 0, r.stderr
+    return str(exe)
+
+
+def _run_reader(exe, mode, D, cplx, fin, tmp_path):
+    out = tmp_path / (mode + ".out")
+    r = __import__("subprocess").run([exe, mode, str(D), str(int(cplx)), str(fin), str(out)],
+                                     capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = out.read_bytes()
+    nb_psi, nb_tot, version = np.frombuffer(raw[:12], dtype=np.int32)
+    pos = 12
+    ind = None
+    if mode != "rst":
+        ind = np.frombuffer(raw[pos:pos + 4 * (D + 2) * nb_tot], dtype=np.int32).reshape(nb_tot, D + 2)
+        pos += 4 * (D + 2) * nb_tot
+    per = 1 + (2 if cplx else 1) * nb_tot
+    vals = np.frombuffer(raw[pos:], dtype=np.float64).reshape(nb_psi, per)
+    T = vals[:, 0]
+    v = vals[:, 1:]
+    if cplx:
+        v = v[:, 0::2] + 1j * v[:, 1::2]
+    return int(nb_psi), int(nb_tot), int(version), ind, T, v
+
+
+@pytest.mark.parametrize("formatted", [True, False])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_version1_files_read_by_fortran_runtime(tmp_path, fortran_reader, formatted, cplx):
+    """Version-1 files written here, read back by amdflang's runtime with the
+    statements of the reference's writer (namelist &headerFile, list-directed
+    index lines and 'T, vec(:)'; or the unformatted sequential records):
+    every count, index tuple, T and coefficient comes back bit for bit.
+    (The reference's own reader, Read_psi_nDBasis case 1,
+    sub_module_psi_io.f90:797-812, reads the vector without the T its writer
+    puts first, :1606-1617; these files follow the writer.)"""
+    b = E.SG4Basis(3, 4)
+    rng = np.random.default_rng(11)
+    psi = rng.standard_normal((b.nb_ba, 3))
+    if cplx:
+        psi = psi + 1j * rng.standard_normal((b.nb_ba, 3))
+    p = tmp_path / "wp1"
+    psi_io.write_psi(str(p), psi, b, version=1, T=0.75, formatted=formatted)
+    nb_psi, nb_tot, version, ind, T, v = _run_reader(
+        fortran_reader, "v1f" if formatted else "v1u", 3, cplx, p, tmp_path)
+    assert (nb_psi, nb_tot, version) == (3, b.nb_ba, 1)
+    assert np.array_equal(ind[:, :3], psi_io.packed_indices(b) + 1)
+    assert np.all(ind[:, 3:] == 1)
+    assert np.all(T == 0.75)
+    assert np.array_equal(v, psi.T)
+
+
+def test_restart_file_read_by_fortran_runtime(tmp_path, fortran_reader):
+    """The propagation restart file read by amdflang's list-directed input as
+    ReadWP_restart reads it: T, nb_WP, nb_tot, then every packet's complex
+    coefficients, bit for bit."""
+    rng = np.random.default_rng(12)
+    psis = rng.standard_normal((4, 37)) + 1j * rng.standard_normal((4, 37))
+    p = tmp_path / "file_WP_restart"
+    psi_io.write_restart(str(p), 12.5, psis)
+    nb_psi, nb_tot, _, _, T, v = _run_reader(fortran_reader, "rst", 0, True, p, tmp_path)
+    assert (nb_psi, nb_tot) == (4, 37)
+    assert np.all(T == 12.5)
+    assert np.array_equal(v, psis)
+
+
+def _dump(path, D, psi, T, ind=None):
+    nb, n = psi.shape
+    parts = [np.array([n, nb, 1], dtype=np.int32).tobytes()]
+    if ind is not None:
+        parts.append(np.ascontiguousarray(ind, dtype=np.int32).tobytes())
+    for j in range(n):
+        v = psi[:, j]
+        if np.iscomplexobj(v):
+            v = np.column_stack([v.real, v.imag]).ravel()
+        parts.append(np.concatenate([[T], v]).astype(np.float64).tobytes())
+    path.write_bytes(b"".join(parts))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_version1_file_written_by_fortran_runtime_is_read(tmp_path, fortran_reader, cplx):
+    """amdflang writes a version-1 file with the reference writer's statements
+    (namelist, list-directed index lines, 'T, vec(:)' in the runtime's own
+    output form); read_psi returns every coefficient and T bit for bit."""
+    b = E.SG4Basis(3, 3)
+    rng = np.random.default_rng(13)
+    psi = rng.standard_normal((b.nb_ba, 2)) * 10.0 ** rng.integers(-30, 30, (b.nb_ba, 2))
+    if cplx:
+        psi = psi + 1j * rng.standard_normal((b.nb_ba, 2))
+    ind = np.hstack([psi_io.packed_indices(b) + 1, np.ones((b.nb_ba, 2), dtype=np.int32)])
+    d, f = tmp_path / "dump", tmp_path / "wp_fortran"
+    _dump(d, 3, psi, -3.25, ind)
+    r = __import__("subprocess").run([fortran_reader, "w1f", "3", str(int(cplx)), str(d), str(f)],
+                                     capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got, info = psi_io.read_psi(str(f), b, cplx=cplx)
+    assert info["version"] == 1 and info["nb_psi"] == 2 and info["T"] == [-3.25, -3.25]
+    assert np.array_equal(got, psi)
+
+
+def test_restart_file_written_by_fortran_runtime_is_read(tmp_path, fortran_reader):
+    rng = np.random.default_rng(14)
+    psis = rng.standard_normal((3, 11)) + 1j * rng.standard_normal((3, 11))
+    d, f = tmp_path / "dump", tmp_path / "file_WP_restart"
+    _dump(d, 0, psis.T, 7.5)
+    r = __import__("subprocess").run([fortran_reader, "wrst", "0", "1", str(d), str(f)],
+                                     capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    T, got = psi_io.read_restart(str(f), 3, 11)
+    assert T == 7.5 and np.array_equal(got, psis)
