@@ -108,3 +108,65 @@ def test_jacobi_5_levels_parameter_gradients_vs_finite_difference(cuda):
         fd = (ev[0] - ev[1]) / (2 * h)
         print(key, dE[:, j], fd)
         assert np.max(np.abs(dE[:, j] - fd) / np.maximum(np.abs(fd), 1e-4)) < 1e-6
+
+
+@pytest.mark.timeout(600)
+def test_jacobi_L10_gradients_hellmann_feynman_and_finite_differences(cuda):
+    """The curvilinear differentiation workload at its size: 3D Jacobi
+    triatomic (type 10, grids evaluated on the device), Smolyak L=10 (2288
+    packed rows).  Gradients of the 5 lowest levels with respect to Morse D,
+    Morse a, the cos^2 coupling and one random direction of the grid values,
+    from the device grid-value gradient contracted with dV/dtheta, against
+      * Hellmann-Feynman <l|dH/dtheta|r>/<l|r>, dH/dtheta applied as an SG4
+        operator of its own (V = dV/dtheta, G = 0): 1e-10 relative;
+      * central differences of the levels, the grids re-evaluated on the
+        device at theta +- h on the same basis: 1e-6 relative."""
+    import time
+    t0 = time.time()
+    L, nlev = 10, 5
+    base = dict(D=0.2, a=1.0, coupling=0.01)
+    op = P.jacobi_triatomic_operator(L, max_nvec=64, **base)
+    b = op.basis
+    print(f"L={L} nb_ba={b.nb_ba} sum_nq={b.sum_nq}", flush=True)
+    q = np.concatenate([b.term_grid(t) for t in range(b.nterms)])
+    R, r, th = q.T
+    ex = lambda x, xe: np.exp(-(x - xe))
+    dV = {"D": (1 - ex(R, 3.0)) ** 2 + (1 - ex(r, 2.0)) ** 2,
+          "a": (2 * 0.2 * (1 - ex(R, 3.0)) * ex(R, 3.0) * (R - 3.0)
+                + 2 * 0.2 * (1 - ex(r, 2.0)) * ex(r, 2.0) * (r - 2.0)),
+          "coupling": np.cos(th) ** 2,
+          "grid": np.random.default_rng(41).uniform(0.0, 1.0, b.sum_nq)}
+    keys = list(dV)
+    # eigenvalue_gradients, with the eigenvectors kept for the check below
+    Ev, Rv, Lv = lowest_eigenpairs(dense_matrix(op), nlev)
+    g = grad_V(op, Lv, Rv)
+    dE = np.stack([g.T @ dV[k] for k in keys], axis=1)
+    print(f"levels {Ev} ({time.time() - t0:.1f} s)", flush=True)
+    # Hellmann-Feynman with the derivative operator built on the product path
+    _, _, Jac, sqRJ = P.jacobi_triatomic_grids(b, **base)
+    for j, k in enumerate(keys):
+        opd = E.SG4Operator(b, 64, potential=dV[k], type_op=10, G=np.zeros(9 * b.sum_nq),
+                            Jac=Jac, sqRJ=sqRJ)
+        hf = np.einsum("iv,iv->v", Lv, dense_matrix(opd) @ Rv)
+        print(k, "grad", dE[:, j], "HF", hf, flush=True)
+        assert np.max(np.abs(dE[:, j] - hf)) < 1e-10 * np.max(np.abs(hf))
+    # central differences (levels 1 and 2 are nearly degenerate: small steps)
+    V0, G0, _, _ = P.jacobi_triatomic_grids(b, **base)
+    for j, k in enumerate(keys):
+        h = 1e-7
+        ev = []
+        for s in (+1, -1):
+            if k == "grid":
+                o2 = E.SG4Operator(b, 64, potential=V0 + s * h * dV[k], type_op=10, G=G0,
+                                   Jac=Jac, sqRJ=sqRJ)
+            else:
+                # same basis (its HO scaling is set by D and a), grids
+                # re-evaluated on the device at the shifted parameter
+                kw = dict(base)
+                kw[k] = base[k] + s * h
+                o2 = E.SG4Operator(b, 64, potential=("jacobi_triatomic", (
+                    P.AMU, P.AMU, kw["D"], kw["a"], 3.0, 2.0, kw["coupling"])))
+            ev.append(np.sort(np.linalg.eigvals(dense_matrix(o2)).real)[:nlev])
+        fd = (ev[0] - ev[1]) / (2 * h)
+        print(k, "fd", fd, f"({time.time() - t0:.1f} s)", flush=True)
+        assert np.max(np.abs(dE[:, j] - fd) / np.maximum(np.abs(fd), 1e-4)) < 1e-6
