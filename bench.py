@@ -160,7 +160,7 @@ def propagation_bench(E, torch, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--prec", default="f32x2", choices=list(PREC),
                     help="precision of the reported value; FP64 is timed beside it "
