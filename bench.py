@@ -40,6 +40,7 @@ sys.path.insert(0, ROOT)
 
 # peak of the unit each mode's term kernel computes on (TFLOP/s, MI355X dense)
 PEAK = {"fp64": 78.6, "f32": 157.3, "f32x2": 2500.0}
+PEAK_HBM_GBPS = 8000.0
 UNIT = {"fp64": "FP64 VALU (= FP64 MFMA peak)", "f32": "FP32 VALU",
         "f32x2": "bf16 MFMA (dense)"}
 # sg4_plan_autotune's best launches for 10D L=8 block 16 on MI355X:
@@ -177,6 +178,7 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (x5), for scaling HBM counters")
     ap.add_argument("--no-other", action="store_true", help="do not time the other precision")
+    ap.add_argument("--no-graph", action="store_true", help="do not time the graph-replayed apply")
     ap.add_argument("--propagation", type=int, default=0, metavar="NSTEPS",
                     help="only time the batched propagation workload: 64 seeded uneven "
                          "Henon-Heiles instances, 200-segment field, NSTEPS Chebyshev steps of "
@@ -269,6 +271,23 @@ def main():
         if oname == "fp64":   # H.psi error of the reported mode against FP64, same input
             e2 = (torch.linalg.norm(Yrep - Y, dim=1) / torch.linalg.norm(Y, dim=1)).max().item()
             other["hpsi_rel_err_2norm_of_reported_vs_fp64"] = e2
+    # the apply replayed from one captured HIP graph, in the rows layout the
+    # device-side drivers use (kinetic rows and term kernel in one graph)
+    graph = None
+    if not args.no_graph:
+        Xr = X.contiguous()                        # (nb_ba, nvec): psi[row*nvec + v]
+        Yr = torch.empty_like(Xr)
+        graph = {"layout": "rows", "steps": args.steps}
+        for pname in dict.fromkeys([args.prec] + ([oname] if other is not None else [])):
+            choose_launch(pname)
+            op.apply_graph(Xr, Yr, PREC[pname])    # capture (not timed)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                op.apply_graph(Xr, Yr, PREC[pname])
+            torch.cuda.synchronize()
+            tg = time.perf_counter() - t0
+            graph[pname] = {"H·psi/s": nvec * args.steps / tg, "ms_per_step": tg / args.steps * 1e3}
     # flops of the reference's algorithm (B->G, kinetic on the grid, G->B):
     # F1 = 3 sum_t S_t per vector, S_t = sum_k 2 N_t n_k
     lev = b.levels()
@@ -290,6 +309,17 @@ def main():
             "F1_reference_flops_per_apply": F1,
             "F1_over_kernel_TFLOPs": F1 / kavg / 1e12}
     if args.prec == "f32x2":
+        # the split kernel is bound by the memory system (its scatter's
+        # memory-side FP64 atomics), not by the matrix cores: the roofline is
+        # the algorithmic bytes against the HBM peak, the flop rates beside it
+        roof.update({"bound": "hbm", "achieved": st["bytes"] / kavg / 1e9, "peak": PEAK_HBM_GBPS,
+                     "unit": "GB/s", "frac": st["bytes"] / kavg / 1e9 / PEAK_HBM_GBPS,
+                     "limit": "memory-side FP64 atomics of the scatter (one per term, packed row "
+                              "and vector); the contractions take ~20 % less time without them",
+                     "atomic_adds_per_launch": b.sum_nb * nvec,   # every map entry (LB = L)
+                     "traffic_GBps": (traffic / kavg / 1e9) if traffic else None,
+                     "algorithmic_TFLOPs": achieved,
+                     "algorithmic_frac_of_bf16_peak": achieved / PEAK["f32x2"]})
         roof["mfma_executed_flops_per_launch"] = mf
         roof["mfma_executed_TFLOPs"] = mf / kavg / 1e12
         roof["mfma_executed_frac_of_peak"] = mf / kavg / 1e12 / PEAK["f32x2"]
@@ -315,6 +345,8 @@ def main():
         "reference_algorithm": {"flops_per_apply": F1,
                                 "effective_TFLOPs": F1 / (dt / args.steps) / 1e12},
     }
+    if graph is not None:
+        line["graph_replay"] = graph
     if other is not None:
         line["other_precision"] = other
         line[f"speedup_vs_{oname}"] = (nvec * args.steps / dt) / other["H·psi/s"]

@@ -286,6 +286,45 @@ def test_graph_replays_match_eager(cuda):
         assert torch.max(torch.abs(Z - (r + 1) * ref)).item() <= 1e-12 * torch.max(torch.abs(Z)).item()
 
 
+def test_graph_replay_10d_L8_block16(cuda):
+    """The headline block through one captured HIP graph (kinetic rows, term
+    kernel and, with the deterministic sum, its clear and finish kernels):
+    FP64 replays with eager work between them equal the eager apply bit for
+    bit; the split-precision replays equal their eager apply to FP64 rounding
+    of the atomic sum (max-norm 1e-10 relative)."""
+    import torch
+    D, L, nvec = 10, 8, 16
+    b = E.SG4Basis(D, L)
+    op = E.SG4Operator(b, nvec, potential=("henon_heiles", 0.111803))
+    g = torch.Generator(device=cuda).manual_seed(31)
+    X = torch.randn(b.nb_ba, nvec, device=cuda, dtype=torch.float64, generator=g)
+    op.set_deterministic(True)
+    ref = torch.empty_like(X)
+    op.apply_rows(X, ref)
+    Y = torch.empty_like(X)
+    for r in range(3):
+        Y.fill_(float("nan"))
+        op.apply_graph(X, Y)
+        torch.cuda.synchronize()
+        assert torch.equal(Y, ref)
+        Z = torch.empty_like(X)
+        op.apply_rows(2.0 * X, Z)                  # eager work between replays
+        torch.cuda.synchronize()
+        # (each contribution is truncated at 2^-64, which does not commute
+        # with the doubling: equal to rounding, not bitwise)
+        assert torch.max(torch.abs(Z - 2.0 * ref)).item() <= 1e-13 * torch.max(torch.abs(Z)).item()
+    op.set_deterministic(False)
+    ref32 = torch.empty_like(X)
+    op.apply_rows(X, ref32, E.PREC_F32X2)
+    for r in range(3):
+        Y.fill_(float("nan"))
+        op.apply_graph(X, Y, E.PREC_F32X2)
+        torch.cuda.synchronize()
+        err = torch.max(torch.abs(Y - ref32)).item() / torch.max(torch.abs(ref32)).item()
+        assert err < 1e-10
+    assert (torch.linalg.norm(ref32 - ref) / torch.linalg.norm(ref)).item() < 2e-6
+
+
 @pytest.mark.parametrize("D,L,prec", [(6, 6, E.PREC_FP64), (4, 6, E.PREC_FP64), (6, 6, E.PREC_F32)])
 def test_grouped_and_per_mode_passes_agree(cuda, D, L, prec):
     """Register-blocked mode groups (default) and one pass per mode give the
