@@ -54,13 +54,20 @@ KERNEL = {"fp64": "k_hpsi_t1s_f64", "f32": "k_hpsi_t1s_f32", "f32x2": "k_hpsi_mf
 
 
 def cpu_model():
+    """'model name' of /proc/cpuinfo with the family/model/stepping numbers
+    (a virtualised host may report only the brand)."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("model name", "cpu family", "model", "stepping") and k not in info:
+                info[k] = v.strip()
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    return (f"{name} (family {info.get('cpu family', '?')}, model {info.get('model', '?')}, "
+            f"stepping {info.get('stepping', '?')}; {os.cpu_count()} logical CPUs visible)")
 
 
 def cpu_baseline(D, L, nvec, stride, min_seconds=10.0):
