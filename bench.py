@@ -316,13 +316,17 @@ def main():
             "F1_reference_flops_per_apply": F1,
             "F1_over_kernel_TFLOPs": F1 / kavg / 1e12}
     if args.prec == "f32x2":
-        # the split kernel is bound by the memory system (its scatter's
-        # memory-side FP64 atomics), not by the matrix cores: the roofline is
-        # the algorithmic bytes against the HBM peak, the flop rates beside it
+        # the split kernel's algorithmic intensity (flops over compulsory
+        # bytes) is below the bf16 ridge (peak flops over peak HBM bandwidth),
+        # so its roofline is the HBM one: algorithmic bytes against the HBM
+        # peak, the flop rates beside it
         roof.update({"bound": "hbm", "achieved": st["bytes"] / kavg / 1e9, "peak": PEAK_HBM_GBPS,
                      "unit": "GB/s", "frac": st["bytes"] / kavg / 1e9 / PEAK_HBM_GBPS,
-                     "limit": "memory-side FP64 atomics of the scatter (one per term, packed row "
-                              "and vector); the contractions take ~20 % less time without them",
+                     "intensity_flop_per_byte": st["flops"] / st["bytes"],
+                     "ridge_flop_per_byte": PEAK["f32x2"] * 1e12 / (PEAK_HBM_GBPS * 1e9),
+                     "limit": "latency of the LDS passes (one dependency chain per tile, a "
+                              "barrier per pass, 4 waves per SIMD); the scatter's memory-side "
+                              "FP64 atomics add ~20 %",
                      "atomic_adds_per_launch": b.sum_nb * nvec,   # every map entry (LB = L)
                      "traffic_GBps": (traffic / kavg / 1e9) if traffic else None,
                      "algorithmic_TFLOPs": achieved,
