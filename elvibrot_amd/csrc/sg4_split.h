@@ -30,8 +30,6 @@ struct SCtx {
   const int32_t* Ms;     // the term's map, staged in LDS
   const double* psi;     // gather: slabs read from the packed block (group 0 only)
   double* hpsi;          // scatter: slabs added into the packed result (group 0 only)
-  unsigned long long* fx;       // deterministic accumulator (or nullptr)
-  unsigned long long* fxflag;
   double w;
   int nvec, v0;
   int gather, scatter;
@@ -86,11 +84,7 @@ __device__ __forceinline__ void split_pass(const SCtx<T>& c) {
 #pragma unroll
       for (int J = 0; J < P; ++J) {
         const int m = c.Ms[P * b + J];
-        if (m >= 0) {
-          const int64_t r = (int64_t)m * c.nvec + c.v0 + av;
-          if (c.fx) fx_add(c.fx + 2 * r, c.fxflag, c.w * (double)x[J]);
-          else atomicAdd(&c.hpsi[r], c.w * (double)x[J]);
-        }
+        if (m >= 0) atomicAdd(&c.hpsi[(int64_t)m * c.nvec + c.v0 + av], c.w * (double)x[J]);
       }
     } else {
 #pragma unroll
@@ -165,8 +159,6 @@ __device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km
   c.Ms = Ms;
   c.psi = a.psi;
   c.hpsi = a.hpsi;
-  c.fx = a.fx;
-  c.fxflag = a.fxflag;
   c.w = w;
   c.nvec = a.nvec;
   c.v0 = it.v0;
@@ -201,7 +193,10 @@ __device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km
     c.invAV = 1.0f / (float)c.AV;
     c.nfib = NE / P;
   };
-  // group 0 gathers; the last grid->basis pass (group 0) scatter-adds
+  // group 0 gathers; the last grid->basis pass (group 0) scatter-adds (FP64
+  // atomics), or, with the deterministic accumulator, leaves its output in X
+  // for one fixed-point pass below (kept out of the unrolled shape passes)
+  const int scat = (a.fx == nullptr);
   for (int g = 0; g + 1 < G; ++g) {
     setup(g, false, false);
     c.gather = (g == 0);
@@ -211,14 +206,23 @@ __device__ __forceinline__ void term_t1_split(const KArgs& a, const KMats<T>& km
   }
   setup(G - 1, true, false);
   c.gather = (G == 1);
-  c.scatter = (G == 1);
+  c.scatter = (G == 1) && scat;
   dispatch_split<T, true>(c, Td->gshape[G - 1]);
   for (int g = G - 2; g >= 0; --g) {
     __syncthreads();
     setup(g, false, true);
     c.gather = 0;
-    c.scatter = (g == 0);
+    c.scatter = (g == 0) && scat;
     dispatch_split<T, false>(c, Td->gshape[g]);
+  }
+  if (!scat) {
+    // group 0's layout: X[av + nv p] is grid point p, vector av
+    __syncthreads();
+    for (int e = threadIdx.x; e < NE; e += nt) {
+      const int m = Ms[e >> lg];
+      if (m >= 0)
+        fx_add(a.fx + 2 * ((int64_t)m * a.nvec + it.v0 + (e & (nv - 1))), a.fxflag, w * (double)X[e]);
+    }
   }
 }
 

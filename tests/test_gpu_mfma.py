@@ -74,6 +74,26 @@ def test_f32x2_general_potential_absent_rows(cuda):
     assert e2 < TOL2 and rc < KCOND
 
 
+def test_f32x2_batch_of_uneven_instances(cuda):
+    """One batched plan over instances of different D and L (the layout of the
+    64-instance propagation batch): each instance's rows of the split-precision
+    H.psi against the oracle of that instance, at the tolerance above."""
+    specs = [(4, 5), (6, 6), (5, 4), (8, 5), (3, 8)]
+    bases = [E.SG4Basis(D, L) for D, L in specs]
+    op = E.SG4Batch(bases, max_nvec=4)
+    assert op.split_info()[0]
+    psi = np.random.default_rng(12).standard_normal((op.basis.nb_ba, 4))
+    got = op.apply_host(psi, E.PREC_F32X2)
+    for (D, L), r0, r1 in zip(specs, op.row0[:-1], op.row0[1:]):
+        o = O.SG4(D, L)
+        V = o.grid_hh(0.111803)
+        ref = o.hpsi(psi[r0:r1], V, -0.5 * np.ones(D))
+        e2, em = errs(got[r0:r1], ref)
+        rc = cond_ratio(got[r0:r1], ref, o, psi[r0:r1], V, D)
+        print(f"instance D={D} L={L}: 2-norm {e2:.2e} max {em:.2e} ratio {rc:.3f}")
+        assert e2 < TOL2 and rc < KCOND
+
+
 def test_f32x2_10d_L8_block16_against_fp64(cuda):
     """The headline workload: every vector of the 16-block against the FP64
     apply (itself at 1e-12 of the oracle) in the 2-norm, two of them against
