@@ -25,7 +25,7 @@ EXPORTS = [
     "sg4_plan_kernel_times", "sg4_grad_V", "sg4_plan_set_field", "sg4_plan_autotune", "sg4_plan_config", "sg4_plan_set_config",
     "sg4_apply_graph", "sg4_plan_set_option", "sg4_plan_split_info", "sg4_plan_mfma_stats",
     "sg4_plan_destroy", "sg4_apply_noalloc", "sg4_cheby_propagate", "sg4_cheby_field_gradient",
-    "sg4_plan_grid_dipole", "sg4_plan_set_operator_type1",
+    "sg4_plan_grid_dipole", "sg4_plan_set_operator_type1", "sg4_taylor_propagate",
 ]
 
 
@@ -50,6 +50,12 @@ class ChebyParams(C.Structure):
                 ("ncoef", C.c_int32), ("coef", D_P), ("nseg", C.c_int32),
                 ("steps_per_seg", C.c_int32), ("eps", D_P), ("ckpt_every", C.c_int32),
                 ("prec", C.c_int32)]
+
+
+class TaylorParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("E0", C.c_double), ("npoly", C.c_int32),
+                ("poly_tol", C.c_double), ("nseg", C.c_int32), ("steps_per_seg", C.c_int32),
+                ("eps", D_P), ("max_norm2", C.c_double), ("prec", C.c_int32)]
 
 
 class Operator(C.Structure):
@@ -105,6 +111,8 @@ def lib():
         L.sg4_cheby_field_gradient.argtypes = [vp, vp, vp, i32, vp, vp, i32,
                                                C.POINTER(C.c_int64), C.POINTER(ChebyParams),
                                                C.POINTER(C.c_double), D_P]
+        L.sg4_taylor_propagate.argtypes = [vp, vp, i32, vp, C.POINTER(TaylorParams),
+                                           C.POINTER(i32)]
         _lib = L
     return _lib
 

@@ -209,6 +209,60 @@ __global__ void k_cb_cscale_rows(const double* __restrict__ x, double* __restric
   }
 }
 
+// One order of the Taylor series (march_noD_field) on a block of width 2 ns:
+//   wn = (-i a) (y - E0 wk),  a = dt / (j + 1),  psi += wn
+// with (-i a)(zr + i zi) = a zi - i a zr.
+__global__ void k_tl_order(const double* __restrict__ wk, const double* __restrict__ y,
+                           double* __restrict__ wn, double* __restrict__ psi, double a, double E0,
+                           int64_t rows, int ns) {
+  const int64_t n = rows * ns;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / ns;
+    const int s = (int)(i - r * ns);
+    const int64_t re = r * 2 * ns + s, im = re + ns;
+    const double zr = y[re] - E0 * wk[re], zi = y[im] - E0 * wk[im];
+    const double nr = a * zi, ni = -a * zr;
+    wn[re] = nr;
+    wn[im] = ni;
+    psi[re] += nr;
+    psi[im] += ni;
+  }
+}
+
+// psi <- (pr + i pi) psi in place
+__global__ void k_tl_phase(double* psi, double pr, double pi, int64_t rows, int ns) {
+  const int64_t n = rows * ns;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / ns;
+    const int s = (int)(i - r * ns);
+    const int64_t re = r * 2 * ns + s, im = re + ns;
+    const double ar = psi[re], ai = psi[im];
+    psi[re] = ar * pr - ai * pi;
+    psi[im] = ar * pi + ai * pr;
+  }
+}
+
+// squared norm of every state of a width-2 ns block: partial sums of state s
+// (grid row blockIdx.y) into part[s * gridDim.x + blockIdx.x]
+constexpr int NRM_BLOCKS = 64;
+__global__ __launch_bounds__(256) void k_tl_norm2(const double* __restrict__ w, int64_t rows, int ns,
+                                                  double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int s = blockIdx.y;
+  double acc = 0.0;
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+    const double xr = w[r * 2 * ns + s], xi = w[r * 2 * ns + ns + s];
+    acc += xr * xr + xi * xi;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[s * gridDim.x + blockIdx.x] = sh[0];
+}
+
 struct Prop {
   sg4_plan* H;
   sg4_plan* Mu;          // the dipole operator (potential mu, no kinetic part)
@@ -307,6 +361,13 @@ struct Prop {
   int field(int s) { return sg4_plan_set_field(H, mu, s < 0 ? 0.0 : p->eps[s]); }
 };
 
+// Restores the field-free potential of H when a propagation leaves on an
+// error path (the normal end restores it itself and clears H).
+struct FieldReset {
+  sg4_plan* H;
+  ~FieldReset() { if (H) (void)sg4_plan_set_field(H, nullptr, 0.0); }
+};
+
 int check_params(const sg4_plan* H, int32_t ns, const sg4_cheby_params* p, int width) {
   if (!H || !p || !p->coef || !p->eps) { set_error("null argument"); return SG4_ERR_ARG; }
   if (ns < 1 || width * ns > plan_max_nvec(H)) {
@@ -338,11 +399,13 @@ int sg4_cheby_propagate(sg4_plan* H, const double* mu, int32_t ns, double* psi,
   P.H = H; P.Mu = nullptr; P.mu = mu; P.ns = ns; P.rows = plan_rows(H);
   P.n = P.rows * 2 * ns; P.st = (hipStream_t)plan_stream(H); P.p = prm;
   if ((rc = P.init(false))) return rc;
+  FieldReset reset{H};
   for (int s = 0; s < prm->nseg; ++s) {
     if ((rc = P.field(s))) return rc;
     for (int j = 0; j < prm->steps_per_seg; ++j)
       if ((rc = P.step(psi, -1.0))) return rc;
   }
+  reset.H = nullptr;
   if ((rc = sg4_plan_set_field(H, nullptr, 0.0))) return rc;
   CB_TRY(hipStreamSynchronize(P.st));
   return SG4_OK;
@@ -382,6 +445,7 @@ int sg4_cheby_field_gradient(sg4_plan* H, sg4_plan* Mu, const double* mu, int32_
     ~Restore() { (void)sg4_plan_set_stream(m, s); }
   } restore{Mu, mu_stream};
   if ((rc = P.init(true))) return rc;
+  FieldReset reset{H};
   const dim3 b(256);
   // forward, keeping the state at the start of every C-th segment
   std::vector<Buf> ckpt((S + C - 1) / C);
@@ -440,8 +504,95 @@ int sg4_cheby_field_gradient(sg4_plan* H, sg4_plan* Mu, const double* mu, int32_
     for (int j = 0; j < prm->steps_per_seg; ++j)
       if ((rc = P.step(lam.p, +1.0))) return rc;
   }
+  reset.H = nullptr;
   if ((rc = sg4_plan_set_field(H, nullptr, 0.0))) return rc;
   CB_TRY(hipStreamSynchronize(P.st));
+  return SG4_OK;
+}
+
+int sg4_taylor_propagate(sg4_plan* H, const double* mu, int32_t ns, double* psi,
+                         const sg4_taylor_params* prm, int32_t* orders) {
+  if (!H || !prm || !prm->eps || !psi) { set_error("null argument"); return SG4_ERR_ARG; }
+  if (ns < 1 || 2 * ns > plan_max_nvec(H)) {
+    set_error("ns out of range: the plan's max_nvec must hold 2 ns vectors");
+    return SG4_ERR_ARG;
+  }
+  if (prm->npoly < 1 || prm->nseg < 1 || prm->steps_per_seg < 1 || !(prm->dt > 0) ||
+      !(prm->poly_tol > 0) || !(prm->max_norm2 >= 0)) {
+    set_error("bad propagation parameters");
+    return SG4_ERR_ARG;
+  }
+  int rc;
+  if ((rc = device_block(H, psi, "psi")) || (mu && (rc = device_block(H, mu, "mu")))) return rc;
+  const int64_t rows = plan_rows(H), n = rows * 2 * ns;
+  const hipStream_t st = (hipStream_t)plan_stream(H);
+  Buf A, B, Y, part;
+  if ((rc = A.alloc(n)) || (rc = B.alloc(n)) || (rc = Y.alloc(n)) ||
+      (rc = part.alloc((size_t)NRM_BLOCKS * ns)))
+    return rc;
+  std::vector<double> hpart((size_t)NRM_BLOCKS * ns);
+  // largest squared norm over the states of a width-2 ns block (synchronises)
+  auto max_norm2 = [&](const double* w, double* out) -> int {
+    hipLaunchKernelGGL(k_tl_norm2, dim3(NRM_BLOCKS, ns), dim3(256), 0, st, w, rows, ns, part.p);
+    CB_TRY(hipGetLastError());
+    CB_TRY(hipMemcpyAsync(hpart.data(), part.p, sizeof(double) * hpart.size(), hipMemcpyDeviceToHost, st));
+    CB_TRY(hipStreamSynchronize(st));
+    double m = 0.0;
+    for (int s = 0; s < ns; ++s) {
+      double a = 0.0;
+      for (int i = 0; i < NRM_BLOCKS; ++i) a += hpart[(size_t)s * NRM_BLOCKS + i];
+      m = (a > m || a != a) ? a : m;
+    }
+    *out = m;
+    return SG4_OK;
+  };
+  const dim3 g(blocks_for(rows * ns)), b(256);
+  const double ph = -prm->E0 * prm->dt;
+  FieldReset reset{H};
+  int64_t step = 0;
+  for (int s = 0; s < prm->nseg; ++s) {
+    if ((rc = sg4_plan_set_field(H, mu, prm->eps[s]))) return rc;
+    for (int j = 0; j < prm->steps_per_seg; ++j, ++step) {
+      hipLaunchKernelGGL(k_cb_copy, dim3(blocks_for(n)), b, 0, st, psi, A.p, n);
+      double* wk = A.p;
+      double* wn = B.p;
+      double m = 0.0;
+      int used = 0;
+      for (int k = 0; k < prm->npoly; ++k) {
+        if ((rc = sg4_apply_rows(H, 2 * ns, wk, Y.p, prm->prec))) return rc;
+        hipLaunchKernelGGL(k_tl_order, g, b, 0, st, wk, Y.p, wn, psi, prm->dt / (k + 1), prm->E0, rows, ns);
+        if ((rc = max_norm2(wn, &m))) return rc;
+        used = k + 1;
+        if (m < prm->poly_tol) break;
+        if (!(m <= 1e15)) {
+          set_error("Taylor propagation: the squared norm of term " + std::to_string(used) + " is " +
+                    std::to_string(m) + " (> 1e15): reduce the time step");
+          return SG4_ERR_STATE;
+        }
+        std::swap(wk, wn);
+      }
+      if (!(m < prm->poly_tol)) {
+        set_error("Taylor propagation: the series did not reach poly_tol within npoly = " +
+                  std::to_string(prm->npoly) + " orders (last squared norm " + std::to_string(m) + ")");
+        return SG4_ERR_STATE;
+      }
+      hipLaunchKernelGGL(k_tl_phase, g, b, 0, st, psi, std::cos(ph), std::sin(ph), rows, ns);
+      CB_TRY(hipGetLastError());
+      if (prm->max_norm2 > 0) {
+        double pn;
+        if ((rc = max_norm2(psi, &pn))) return rc;
+        if (!(pn <= prm->max_norm2)) {
+          set_error("Taylor propagation: a state's squared norm " + std::to_string(pn) +
+                    " exceeds max_norm2 after step " + std::to_string(step + 1));
+          return SG4_ERR_STATE;
+        }
+      }
+      if (orders) orders[step] = used;
+    }
+  }
+  reset.H = nullptr;
+  if ((rc = sg4_plan_set_field(H, nullptr, 0.0))) return rc;
+  CB_TRY(hipStreamSynchronize(st));
   return SG4_OK;
 }
 

@@ -1,6 +1,6 @@
-"""Chebyshev propagation on the device around the SG4 H.psi library, with
-piecewise-constant control fields and the exact gradient of a final-state
-overlap with respect to the field amplitudes.
+"""Chebyshev and Taylor propagation on the device around the SG4 H.psi
+library, with piecewise-constant control fields, and the exact gradient of a
+final-state overlap with respect to the field amplitudes (Chebyshev).
 
 Chebyshev step (march_cheby, SRC/sub_propagation/sub_module_propa_march.f90
 :4145-4436): with Hs = (H - E0)/Esc, E0 = (Hmin+Hmax)/2, Esc = (Hmax-Hmin)/2,
@@ -15,6 +15,11 @@ real and imaginary parts of all states form one block for the H.psi kernel.
 Field: H(t) = H0 + eps_s mu on segment s (mu = sum_i q_i for the dipole
 coupling); the reference has field-driven Taylor/RK propagators only
 (march_noD_field, :1641), Chebyshev per constant-field segment is new here.
+
+Taylor step (march_noD_field, :1641-1842, the field constant on a segment):
+w_0 = psi, w_{k+1} = (-i dt/(k+1)) (H - E0) w_k, psi += w_{k+1} until the
+squared norm of the new term is below poly_tol (default 1e-20), then
+psi *= exp(-i E0 dt); see propagate_taylor_device.
 
 Gradient of J = |<target|psi(T)>|^2 with respect to eps_s: exact for the
 discrete map.  The adjoint state lambda_s = U_{>s}^dagger target is propagated
@@ -315,3 +320,48 @@ def field_gradient_device(prop, psi0, target, eps, steps_per_segment, checkpoint
         C.c_void_p(tg.data_ptr()), len(blocks), br, C.byref(prm), C.byref(J),
         grad.ctypes.data_as(_capi.D_P)))
     return J.value, grad
+
+
+# ---- Taylor propagation (march_noD_field) --------------------------------------
+def taylor_npoly(alpha, poly_tol=1e-20, max_poly=5000):
+    """Maximal Taylor order for alpha = (Hmax - Hmin) dt / 2 (cof_nOD,
+    SRC/sub_propagation/sub_module_propa_march.f90:5771-5827): first the order
+    at which log(sum_k<=i alpha^k/k!) reaches alpha within poly_tol, then on
+    until the term alpha^i/i! is below poly_tol, both at most max_poly + 1."""
+    i, xi, reste = 1, float(alpha), 1.0 + alpha
+    while abs(math.log(reste) - alpha) > poly_tol and i <= max_poly:
+        i += 1
+        xi *= alpha / i
+        reste += xi
+    while xi > poly_tol and i <= max_poly:
+        i += 1
+        xi *= alpha / i
+    return i
+
+
+def propagate_taylor_device(op, psi0, dt, Hmin, Hmax, eps=(0.0,), steps_per_segment=1, mu=None,
+                            poly_tol=1e-20, npoly=None, max_norm2=0.0, prec=PREC_FP64,
+                            max_poly=5000):
+    """psi(T) by Taylor steps on the device (sg4_taylor_propagate): the series
+    in H - E0 with E0 = (Hmin + Hmax)/2, cut per step when every state's new
+    term has a squared norm below poly_tol (reference default 1e-20), at most
+    npoly terms (default taylor_npoly of (Hmax - Hmin) dt / 2), H = H0 + eps_s mu
+    on segment s (mu: device grid tensor, or None for no field).  Returns
+    (psi(T), number of terms of every step)."""
+    ns = psi0.shape[0]
+    E0 = Hmin + 0.5 * (Hmax - Hmin)
+    if npoly is None:
+        npoly = taylor_npoly(0.5 * (Hmax - Hmin) * dt, poly_tol, max_poly)
+    ev = np.ascontiguousarray(eps, dtype=np.float64)
+    prm = _capi.TaylorParams()
+    prm.dt, prm.E0, prm.npoly, prm.poly_tol = dt, E0, int(npoly), poly_tol
+    prm.nseg, prm.steps_per_seg = len(ev), steps_per_segment
+    prm.eps = ev.ctypes.data_as(_capi.D_P)
+    prm.max_norm2, prm.prec = max_norm2, prec
+    orders = np.zeros(len(ev) * steps_per_segment, dtype=np.int32)
+    x = to_rows(psi0)
+    mc = mu.contiguous() if mu is not None else None
+    mup = C.c_void_p(mc.data_ptr()) if mc is not None else None
+    _capi.check(lib().sg4_taylor_propagate(op._h, mup, ns, C.c_void_p(x.data_ptr()), C.byref(prm),
+                                           orders.ctypes.data_as(C.POINTER(C.c_int32))))
+    return from_rows(x, ns), orders

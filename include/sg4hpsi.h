@@ -261,6 +261,39 @@ int sg4_cheby_field_gradient(sg4_plan *H, sg4_plan *Mu, const double *mu, int32_
                              const int64_t *blk_rows, const sg4_cheby_params *prm, double *J,
                              double *grad);
 
+/* ---- Taylor propagation under a piecewise-constant field ----------------
+ * march_noD_field (SRC/sub_propagation/sub_module_propa_march.f90:1641-1842)
+ * with the field constant on each segment, so that the field's time
+ * derivatives in its series vanish: H = H0 + eps[s] mu on segment s
+ * (sg4_plan_set_field; the reference's field E enters as eps = -E, :1736), and
+ * per step
+ *   w_0 = psi,  w_{k+1} = (-i dt/(k+1)) (H - E0) w_k,  psi <- psi + w_{k+1}
+ * (:1756-1787), cut after the first order whose squared norm is below
+ * poly_tol for every state (:1791; the reference cuts each state on its own),
+ * then psi <- exp(-i E0 dt) psi (:1812).  Errors (SG4_ERR_STATE), as the
+ * reference's STOPs: a term's squared norm above 1e15 (time step too large,
+ * :1792), no cut within npoly orders (npoly too small, march_noD :1959), a
+ * state's squared norm above max_norm2 after a step when max_norm2 > 0
+ * (:1817).  npoly from the spectral range is the caller's choice (cof_nOD,
+ * :5771-5827).  States as for sg4_cheby_propagate (DEVICE rows layout,
+ * nb_ba x 2 ns values, max_nvec >= 2 ns); orders: host array of
+ * nseg x steps_per_seg entries receiving the number of terms each step used
+ * (or NULL).  One H.psi launch and one host synchronisation per order (the new
+ * term's norm decides the cut).  H0 is restored at the end, also on errors. */
+typedef struct {
+  double dt;              /* time step */
+  double E0;              /* energy shift of the series */
+  int32_t npoly;          /* maximal number of terms per step */
+  double poly_tol;        /* cut: squared norm of a term (reference default 1e-20) */
+  int32_t nseg;           /* field segments */
+  int32_t steps_per_seg;  /* steps per segment */
+  const double *eps;      /* host: nseg field amplitudes */
+  double max_norm2;       /* > 0: bound on a state's squared norm (reference 1.3) */
+  int32_t prec;           /* precision mode of the H.psi applications */
+} sg4_taylor_params;
+int sg4_taylor_propagate(sg4_plan *H, const double *mu, int32_t ns, double *psi,
+                         const sg4_taylor_params *prm, int32_t *orders);
+
 /* Derivative of <phi_v|H|psi_v> with respect to every grid value of the
  * potential: grad[(p)*nvec + v] = W_t w(p) (B_t phi_v)(p) (B_t psi_v)(p) for the
  * grid point p of term t (w(p): product of the 1-D quadrature weights).  With

@@ -246,3 +246,36 @@ def num_threads():
 def henon_heiles(q, lam=0.111803):
     q = np.ascontiguousarray(q, dtype=np.float64)
     return lib().orc_henon_heiles(len(q), _d(q), lam)
+
+
+def taylor_propagate(hfun, psi, dt, E0, npoly, poly_tol, eps=(0.0,), steps_per_segment=1,
+                     per_state=False):
+    """Taylor propagation (march_noD_field,
+    SRC/sub_propagation/sub_module_propa_march.f90:1641-1842) with the field
+    constant on each segment: hfun(X, eps) is the real H.psi of the columns of
+    X under H0 + eps mu (complex states are applied as their real and
+    imaginary parts, sub_OpPsi.f90:397-411).  Per step: w_0 = psi,
+    w_{k+1} = (-i dt/(k+1)) (H - E0) w_k (:1756-1765), psi += w_{k+1} (:1787),
+    cut after the first k whose ||w_{k+1}||^2 < poly_tol (:1791): for every
+    state of the block together, or per state (per_state=True, the reference's
+    loop over WP(j)); psi *= exp(-i E0 dt) (:1812).  psi: complex
+    (nb_ba, ns).  Returns (psi(T), terms used per step and state)."""
+    psi = np.array(psi, dtype=np.complex128)
+    ns = psi.shape[1]
+    used = []
+    for e in eps:
+        for _ in range(steps_per_segment):
+            cols = [np.arange(ns)] if not per_state else [np.array([s]) for s in range(ns)]
+            step_used = np.zeros(ns, dtype=int)
+            for c in cols:
+                w = psi[:, c].copy()
+                for k in range(npoly):
+                    hw = hfun(w.real, e) + 1j * hfun(w.imag, e)
+                    w = (-1j * dt / (k + 1)) * (hw - E0 * w)
+                    psi[:, c] += w
+                    step_used[c] = k + 1
+                    if np.max(np.sum(np.abs(w) ** 2, axis=0)) < poly_tol:
+                        break
+            psi *= np.exp(-1j * E0 * dt)
+            used.append(step_used)
+    return psi, np.array(used)

@@ -208,3 +208,76 @@ def test_device_propagator_and_gradient(cuda, prec):
         rel = np.max(np.abs(g - g64)) / np.max(np.abs(g64))
         print(f"split-precision gradient vs FP64: {rel:.2e}, J {J:.12f} vs {J64:.12f}")
         assert rel < 1e-4 and abs(J - J64) < 1e-5 * J64
+
+
+def _taylor_case(cuda):
+    import torch
+    D, L = 3, 3
+    b, op = hh(D, L)
+    o = O.SG4(D, L)
+    q = np.concatenate([o.term_grid(t) for t in range(o.nterms)]).sum(1)
+    lo, hi = spectral_bounds(op)
+    w = 0.05 * (hi - lo) + 0.05 * np.abs(q).max()
+    rng = np.random.default_rng(11)
+    psi = rng.standard_normal((2, b.nb_ba)) + 1j * rng.standard_normal((2, b.nb_ba))
+    psi /= np.linalg.norm(psi, axis=1)[:, None]
+    return D, b, op, o, q, torch.tensor(q, device=cuda), lo - w, hi + w, psi
+
+
+def test_taylor_device_vs_oracle_exponential_and_chebyshev(cuda):
+    """sg4_taylor_propagate (march_noD_field, field constant per segment)
+    against the oracle's Taylor propagation with the same block cut (same
+    terms per step, 1e-12), the dense exponentials of the oracle matrices
+    (1e-10) and the device Chebyshev propagation of the same field (1e-9)."""
+    import torch
+    from elvibrot_amd.propagation import propagate_taylor_device, propagate_device, taylor_npoly
+    D, b, op, o, q, mu, Hmin, Hmax, psi = _taylor_case(cuda)
+    eps, steps, dt = [0.05, -0.03, 0.02], 2, 0.1
+    got, orders = propagate_taylor_device(op, torch.tensor(psi, device=cuda), dt, Hmin, Hmax, eps,
+                                          steps, mu=mu)
+    got = got.cpu().numpy()
+    E0 = Hmin + 0.5 * (Hmax - Hmin)
+    V, ck = o.grid_hh(0.111803), -0.5 * np.ones(D)
+    npoly = taylor_npoly(0.5 * (Hmax - Hmin) * dt)
+    ref, used = O.taylor_propagate(lambda X, e: o.hpsi(X, V + e * q, ck), psi.T, dt, E0, npoly,
+                                   1e-20, eps, steps)
+    print(f"terms per step: device {orders.tolist()}, oracle {used[:, 0].tolist()}")
+    assert orders.tolist() == used[:, 0].tolist()
+    assert np.max(np.abs(got - ref.T)) < 1e-12
+    H = o.hpsi(np.eye(o.nb_ba), V, ck)
+    M = o.hpsi(np.eye(o.nb_ba), q, np.zeros(D))
+    ex = psi.T.copy()
+    for e in eps:
+        U = sla.expm(-1j * (H + e * M) * dt)
+        for _ in range(steps):
+            ex = U @ ex
+    assert np.max(np.abs(got - ex.T)) < 1e-10
+    assert np.max(np.abs(np.linalg.norm(got, axis=1) - 1)) < 1e-10
+    prop = ChebyshevPropagator(op, dt, Hmin, Hmax, mu=mu)
+    ch = propagate_device(prop, torch.tensor(psi, device=cuda), eps, steps).cpu().numpy()
+    assert np.max(np.abs(got - ch)) < 1e-9
+
+
+def test_taylor_device_errors_restore_the_field(cuda):
+    """The reference's STOPs become SG4_ERR_STATE: series not cut within npoly
+    terms, a term's squared norm above 1e15 (time step too large), a state's
+    squared norm above max_norm2; the field-free operator is back afterwards
+    (to the rounding of the FP64 scatter-adds, whose order varies)."""
+    import torch
+    from elvibrot_amd.propagation import propagate_taylor_device
+    D, b, op, o, q, mu, Hmin, Hmax, psi = _taylor_case(cuda)
+    x = np.random.default_rng(2).standard_normal((b.nb_ba, 2))
+    h0 = op.apply_host(x)
+    t = torch.tensor(psi, device=cuda)
+    same = lambda a, b: np.max(np.abs(a - b)) < 1e-13 * np.max(np.abs(b))
+    with pytest.raises(E.SG4Error, match="poly_tol"):
+        propagate_taylor_device(op, t, 0.1, Hmin, Hmax, [0.3], 1, mu=mu, npoly=3)
+    assert same(op.apply_host(x), h0)
+    with pytest.raises(E.SG4Error, match="reduce the time step"):
+        propagate_taylor_device(op, t, 20.0, Hmin, Hmax, [0.3], 1, mu=mu, npoly=400)
+    assert same(op.apply_host(x), h0)
+    with pytest.raises(E.SG4Error, match="max_norm2"):
+        propagate_taylor_device(op, t, 0.1, Hmin, Hmax, [0.3], 1, mu=mu, max_norm2=0.5)
+    assert same(op.apply_host(x), h0)
+    _, orders = propagate_taylor_device(op, t, 0.1, Hmin, Hmax, [0.3], 1, mu=mu, max_norm2=1.3)
+    assert orders[0] > 5

@@ -399,3 +399,63 @@ def test_type10_nonsymmetric_G_jacobian_and_density_expanded():
     dGT = [[dGs[j][i] for j in range(2)] for i in range(2)]
     assert np.max(np.abs(A - U.T @ expanded(GT, dGT) @ U)) > 1e-2
     assert np.max(np.abs(U.T @ type10(Gs, np.ones(n)) @ U - B)) > 1e-2
+
+
+def _hh_dense(D, L):
+    o = O.SG4(D, L)
+    V = o.grid_hh(0.111803)
+    mu = np.concatenate([o.term_grid(t) for t in range(o.nterms)]).sum(1)
+    ck = -0.5 * np.ones(D)
+    hfun = lambda X, e: o.hpsi(X, V + e * mu, ck)
+    I = np.eye(o.nb_ba)
+    return o, hfun, o.hpsi(I, V, ck), o.hpsi(I, mu, np.zeros(D))
+
+
+def test_taylor_oracle_equals_exponential_under_a_field():
+    """Taylor propagation (march_noD_field) with segment-constant fields equals
+    prod_s exp(-i (H + eps_s M) dt)^steps on the dense matrices of the same
+    operator; a wrong sign of i, a dropped 1/k or a missing E0 phase fails."""
+    import scipy.linalg as sla
+    o, hfun, H, M = _hh_dense(3, 3)
+    ev = np.linalg.eigvalsh(0.5 * (H + H.T))
+    E0, dt, eps = 0.5 * (ev[0] + ev[-1]), 0.1, [0.05, -0.03, 0.02]
+    rng = np.random.default_rng(5)
+    psi = rng.standard_normal((o.nb_ba, 2)) + 1j * rng.standard_normal((o.nb_ba, 2))
+    got, used = O.taylor_propagate(hfun, psi, dt, E0, 400, 1e-28, eps, 2)
+    ref = psi.copy()
+    for e in eps:
+        U = sla.expm(-1j * (H + e * M) * dt)
+        ref = U @ (U @ ref)
+    assert np.max(np.abs(got - ref)) < 1e-12 * np.max(np.abs(ref))
+    assert used.shape == (6, 2) and used.min() > 5
+    bad, _ = O.taylor_propagate(lambda X, e: -hfun(X, e), psi, dt, -E0, 400, 1e-28, eps, 2)
+    assert np.max(np.abs(bad - ref)) > 1e-2
+
+
+def test_taylor_oracle_cut_rule():
+    """Terms used per step = first k with ||(-i dt)^k/k! (H - E0)^k psi||^2 below
+    poly_tol, from the eigen-decomposition (per state, as the reference's loop
+    over the packets; the block rule takes the largest)."""
+    o, hfun, H, _ = _hh_dense(2, 4)
+    Hs = 0.5 * (H + H.T)
+    lam, U = np.linalg.eigh(Hs)
+    E0, dt, tol = 0.5 * (lam[0] + lam[-1]), 0.2, 1e-20
+    rng = np.random.default_rng(6)
+    psi = rng.standard_normal((o.nb_ba, 3)) + 0j
+    psi[:, 0] = U[:, np.argmin(np.abs(lam - 0.5 * (lam[0] + lam[-1])))]   # eigenvalue near E0: fastest
+    sym = lambda X, e: Hs @ X
+    c = U.T @ psi
+    expect = []
+    for s in range(3):
+        k, term = 0, c[:, s].astype(complex)
+        while True:
+            k += 1
+            term = term * (-1j * dt / k) * (lam - E0)
+            if np.sum(np.abs(term) ** 2) < tol:
+                break
+        expect.append(k)
+    _, per = O.taylor_propagate(sym, psi, dt, E0, 500, tol, per_state=True)
+    _, blk = O.taylor_propagate(sym, psi, dt, E0, 500, tol)
+    assert list(per[0]) == expect
+    assert list(blk[0]) == [max(expect)] * 3
+    assert expect[0] < expect[1]
