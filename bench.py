@@ -13,7 +13,8 @@ apply of the same operator is timed beside it with the same number of steps,
 and their ratio is reported.  The eigenvalue error condition of the metric is
 measured by block Davidson runs for the 20 lowest levels: FP64, split
 precision followed by an FP64 phase, and split precision alone, each
-compared with the FP64 levels.
+compared with the FP64 levels; each run is timed after an untimed run of the
+same kind (first-use set-up of the block widths it applies).
 
 One JSON line: the metric, the roofline of the dominant kernel (the term
 kernel's main launch: algorithmic flops per launch over the launch time
@@ -364,9 +365,12 @@ def main():
     if args.nlev and args.prec != "fp64":
         from elvibrot_amd.davidson import sub_propagation_Davidson
         runs = {}
-        for name, p, refine in (("fp64", PREC["fp64"], True),
-                                (f"{args.prec}_then_fp64", prec, True),
-                                (f"{args.prec}_only", prec, False)):
+        legs = (("fp64", PREC["fp64"], True), (f"{args.prec}_then_fp64", prec, True),
+                (f"{args.prec}_only", prec, False))
+        for name, p, refine in legs:     # untimed: first-use set-up of every block width
+            sub_propagation_Davidson(op, args.nlev, max_it=400, conv_resi=1e-6, conv_ene=1e-10,
+                                     prec=p, refine=refine)
+        for name, p, refine in legs:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             r = sub_propagation_Davidson(op, args.nlev, max_it=400, conv_resi=1e-6,

@@ -85,35 +85,37 @@ __global__ void k_to_f32(const double* __restrict__ in, float* __restrict__ out,
 // small kernels
 // ---------------------------------------------------------------------------
 // Layout changes between the column-major block (v * nrow + r) and the rows
-// layout (r * nvec + v), through 64-row x 16-vector LDS tiles so that both the
-// reads and the writes are contiguous.  grid = (ceil(nrow/64), ceil(nvec/16)),
-// 256 threads.
+// layout (r * W + v) of width W >= nvec (columns nvec..W-1 of the rows layout
+// are zero going in and ignored coming out), through 64-row x 16-vector LDS
+// tiles so that both the reads and the writes are contiguous.
+// grid = (ceil(nrow/64), ceil(W/16)), 256 threads.
 constexpr int TT_R = 64, TT_V = 16;
 __global__ __launch_bounds__(256) void k_cols_to_rows(const double* __restrict__ in,
                                                       double* __restrict__ out, int64_t nrow,
-                                                      int nvec) {
+                                                      int nvec, int W) {
   __shared__ double tile[TT_V][TT_R + 1];
   const int64_t r0 = (int64_t)blockIdx.x * TT_R;
   const int v0 = blockIdx.y * TT_V;
   for (int k = threadIdx.x; k < TT_R * TT_V; k += blockDim.x) {
     const int vv = k / TT_R, rr = k % TT_R;   // read columns: rows contiguous
-    if (r0 + rr < nrow && v0 + vv < nvec) tile[vv][rr] = in[(int64_t)(v0 + vv) * nrow + r0 + rr];
+    if (r0 + rr < nrow && v0 + vv < W)
+      tile[vv][rr] = (v0 + vv < nvec) ? in[(int64_t)(v0 + vv) * nrow + r0 + rr] : 0.0;
   }
   __syncthreads();
   for (int k = threadIdx.x; k < TT_R * TT_V; k += blockDim.x) {
     const int rr = k / TT_V, vv = k % TT_V;   // write rows: vectors contiguous
-    if (r0 + rr < nrow && v0 + vv < nvec) out[(r0 + rr) * nvec + v0 + vv] = tile[vv][rr];
+    if (r0 + rr < nrow && v0 + vv < W) out[(r0 + rr) * W + v0 + vv] = tile[vv][rr];
   }
 }
 __global__ __launch_bounds__(256) void k_rows_to_cols(const double* __restrict__ in,
                                                       double* __restrict__ out, int64_t nrow,
-                                                      int nvec) {
+                                                      int nvec, int W) {
   __shared__ double tile[TT_V][TT_R + 1];
   const int64_t r0 = (int64_t)blockIdx.x * TT_R;
   const int v0 = blockIdx.y * TT_V;
   for (int k = threadIdx.x; k < TT_R * TT_V; k += blockDim.x) {
     const int rr = k / TT_V, vv = k % TT_V;   // read rows: vectors contiguous
-    if (r0 + rr < nrow && v0 + vv < nvec) tile[vv][rr] = in[(r0 + rr) * nvec + v0 + vv];
+    if (r0 + rr < nrow && v0 + vv < nvec) tile[vv][rr] = in[(r0 + rr) * W + v0 + vv];
   }
   __syncthreads();
   for (int k = threadIdx.x; k < TT_R * TT_V; k += blockDim.x) {
@@ -121,8 +123,8 @@ __global__ __launch_bounds__(256) void k_rows_to_cols(const double* __restrict__
     if (r0 + rr < nrow && v0 + vv < nvec) out[(int64_t)(v0 + vv) * nrow + r0 + rr] = tile[vv][rr];
   }
 }
-static dim3 tile_grid(int64_t nrow, int nvec) {
-  return dim3((unsigned)((nrow + TT_R - 1) / TT_R), (unsigned)((nvec + TT_V - 1) / TT_V));
+static dim3 tile_grid(int64_t nrow, int W) {
+  return dim3((unsigned)((nrow + TT_R - 1) / TT_R), (unsigned)((W + TT_V - 1) / TT_V));
 }
 __global__ void k_fill(double* __restrict__ p, int64_t n, double val) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -1119,9 +1121,23 @@ static bool use_split(const sg4_plan* P) {
   return P->type_op == 1 && P->grouping && P->split_ok;
 }
 
-static LaunchCfg cfg_for(sg4_plan* P, int nvec, int prec) {
+// Launch configuration for (nvec, prec): the one tuned or set for it, else
+// the one set for the nearest block width of the same precision (the
+// narrowest wider one first): the best launch shape depends on the term set
+// far more than on the block width, so a block of 8 vectors in a Davidson
+// iteration runs with the launch tuned for 16.  Defaults when none is set.
+static LaunchCfg cfg_for(const sg4_plan* P, int nvec, int prec) {
   auto it = P->tuned.find(nvec * 16 + prec);
-  return it == P->tuned.end() ? LaunchCfg{} : it->second;
+  if (it != P->tuned.end()) return it->second;
+  const LaunchCfg* best = nullptr;
+  int bw = 0;
+  for (const auto& kv : P->tuned) {
+    if (kv.first % 16 != prec) continue;
+    const int w = kv.first / 16;
+    const bool better = !best || (w > nvec && (bw < nvec || w < bw)) || (w < nvec && bw < nvec && w > bw);
+    if (better) { best = &kv.second; bw = w; }
+  }
+  return best ? *best : LaunchCfg{};
 }
 
 // Everything an apply needs that may allocate or copy (FP32 copies, item
@@ -1149,6 +1165,23 @@ static int prepare(sg4_plan* P, int nvec, int prec, const LaunchCfg& cfg, ItemPl
   const bool t10 = P->type_op == 10;
   return get_items(P, nvec, esize, ip, 0, t10 ? LDS_SMALL : cfg.lds_budget, t10 ? 0 : cfg.xcd,
                    use_split(P));
+}
+
+// Width of the rows-layout staging block of a column-major apply of nvec
+// vectors.  Work items carry 2^g <= 16 vectors and a term's item costs little
+// more with 16 vectors than with 8 (its passes are latency-bound), so the
+// block is padded with zero vectors to the next power of two (nvec <= 16) or
+// multiple of 16 when the plan's max_nvec allows: 9 vectors, say, then run as
+// one 16-vector item per term instead of an 8- and a 1-vector item (10D L=8:
+// 5.3 instead of 7.8 ms).
+static int pad_width(const sg4_plan* p, int nvec) {
+  int W = 1;
+  if (nvec <= 16) {
+    while (W < nvec) W <<= 1;
+  } else {
+    W = 16 * ((nvec + 15) / 16);
+  }
+  return W <= p->max_nvec ? W : nvec;
 }
 
 static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, int prec,
@@ -1764,16 +1797,16 @@ int sg4_apply(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi, int32_
   if (int rc0 = device_blocks(p, psi, hpsi)) return rc0;
   if (nvec < 1 || nvec > p->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
   if (blocks_overlap(p, nvec, psi, hpsi)) return SG4_ERR_ARG;
-  const int64_t n = p->nb_ba * nvec;
   double* ri = p->d_rows_in.as<double>();
   double* ro = p->d_rows_out.as<double>();
   if (nvec == 1) {
     return apply_rows(p, 1, psi, hpsi, prec);
   }
-  hipLaunchKernelGGL(k_cols_to_rows, tile_grid(p->nb_ba, nvec), dim3(256), 0, p->stream, psi, ri, p->nb_ba, nvec);
-  int rc = apply_rows(p, nvec, ri, ro, prec);
+  const int W = pad_width(p, nvec);
+  hipLaunchKernelGGL(k_cols_to_rows, tile_grid(p->nb_ba, W), dim3(256), 0, p->stream, psi, ri, p->nb_ba, nvec, W);
+  int rc = apply_rows(p, W, ri, ro, prec);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_rows_to_cols, tile_grid(p->nb_ba, nvec), dim3(256), 0, p->stream, ro, hpsi, p->nb_ba, nvec);
+  hipLaunchKernelGGL(k_rows_to_cols, tile_grid(p->nb_ba, W), dim3(256), 0, p->stream, ro, hpsi, p->nb_ba, nvec, W);
   HIP_TRY(hipGetLastError());
   return SG4_OK;
 }
@@ -1812,10 +1845,12 @@ int sg4_apply_noalloc(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi
   if (int rc0 = device_blocks(p, psi, hpsi)) return rc0;
   if (nvec < 1 || nvec > p->max_nvec) { set_error("nvec out of range"); return SG4_ERR_ARG; }
   if (blocks_overlap(p, nvec, psi, hpsi)) return SG4_ERR_ARG;
-  const size_t n = (size_t)p->nb_ba * nvec;
+  const int W = pad_width(p, nvec);
+  const size_t n = (size_t)p->nb_ba * W;
   if (nvec > 1 && 2 * sizeof(double) * n > SG4_SCRATCH_BYTES) {
-    set_error("sg4_apply_noalloc: nb_ba x nvec = " + std::to_string(n) +
-              " exceeds its device scratch (two FP64 blocks in 256 MiB: nb_ba x nvec <= 16777216)");
+    set_error("sg4_apply_noalloc: nb_ba x " + std::to_string(W) + " = " + std::to_string(n) +
+              " exceeds its device scratch (two FP64 blocks in 256 MiB: nb_ba x nvec <= 16777216, "
+              "nvec counted after rounding up to a power of two or a multiple of 16)");
     return SG4_ERR_ARG;
   }
   double* base = nullptr;
@@ -1831,9 +1866,9 @@ int sg4_apply_noalloc(sg4_plan* p, int32_t nvec, const double* psi, double* hpsi
   if (nvec == 1) return apply_rows(p, 1, psi, hpsi, prec);
   double* ri = base;
   double* ro = base + n;
-  hipLaunchKernelGGL(k_cols_to_rows, tile_grid(p->nb_ba, nvec), dim3(256), 0, p->stream, psi, ri, p->nb_ba, nvec);
-  if ((rc = apply_rows(p, nvec, ri, ro, prec))) return rc;
-  hipLaunchKernelGGL(k_rows_to_cols, tile_grid(p->nb_ba, nvec), dim3(256), 0, p->stream, ro, hpsi, p->nb_ba, nvec);
+  hipLaunchKernelGGL(k_cols_to_rows, tile_grid(p->nb_ba, W), dim3(256), 0, p->stream, psi, ri, p->nb_ba, nvec, W);
+  if ((rc = apply_rows(p, W, ri, ro, prec))) return rc;
+  hipLaunchKernelGGL(k_rows_to_cols, tile_grid(p->nb_ba, W), dim3(256), 0, p->stream, ro, hpsi, p->nb_ba, nvec, W);
   HIP_TRY(hipGetLastError());
   return SG4_OK;
 }
@@ -1996,8 +2031,7 @@ int sg4_plan_autotune(sg4_plan* p, int32_t nvec, int32_t prec, float* best_ms) {
 
 int sg4_plan_config(const sg4_plan* p, int32_t nvec, int32_t prec, int32_t* cfg4) {
   if (!p || !cfg4) { set_error("null argument"); return SG4_ERR_ARG; }
-  auto it = p->tuned.find(nvec * 16 + prec);
-  const LaunchCfg c = it == p->tuned.end() ? LaunchCfg{} : it->second;
+  const LaunchCfg c = cfg_for(p, nvec, prec);
   cfg4[0] = c.nthreads; cfg4[1] = c.lds_budget; cfg4[2] = c.grid; cfg4[3] = c.xcd;
   return SG4_OK;
 }

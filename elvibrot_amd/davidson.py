@@ -48,17 +48,19 @@ def ho_zeroth_order_diagonal(basis, device):
 
 def _gram(A, B, chunks=256):
     """A @ B.T for row blocks A (m, n), B (p, n) with a long inner dimension
-    (n = nb_ba): split over `chunks` pieces of n with one batched product plus
-    the remainder.  For small m, p and n ~ 1e6 the library GEMM picks
-    single-tile kernels without a K split that run for ~0.2 s."""
+    (n = nb_ba): one batched product over `chunks` pieces of n (strided views
+    of A and B, no copies) summed over the pieces, plus the remainder.  For
+    small m, p and n ~ 1e6 the library GEMM picks kernels without a K split:
+    A @ B.T takes 2.4 ms for 84 x 84 and 26 ms for 64 x 1 at n = 1.26e6, this
+    form 0.3-0.6 ms for every shape up to 84 x 84."""
     m, n = A.shape
     p = B.shape[0]
     k = n // chunks
     n0 = k * chunks
     out = None
     if k > 0:
-        Ac = A[:, :n0].reshape(m, chunks, k).transpose(0, 1)
-        Bc = B[:, :n0].reshape(p, chunks, k).permute(1, 2, 0).contiguous()
+        Ac = A[:, :n0].reshape(m, chunks, k).transpose(0, 1)      # (chunks, m, k)
+        Bc = B[:, :n0].reshape(p, chunks, k).permute(1, 2, 0)     # (chunks, k, p)
         out = torch.bmm(Ac, Bc).sum(0)
     if n0 < n:
         r = A[:, n0:] @ B[:, n0:].T
@@ -92,6 +94,7 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
     E0 = ho_zeroth_order_diagonal(op.basis, dev)
     Vb = torch.zeros(max_WP, nb, device=dev, dtype=torch.float64)   # basis (rows = vectors)
     HV = torch.zeros_like(Vb)
+    Hp = np.zeros((max_WP, max_WP))                                    # <Vb_i | H Vb_j>
     if psi0 is None:
         order = torch.argsort(E0)[:nb_WP]
         psi0 = torch.zeros(nb_WP, nb, device=dev, dtype=torch.float64)
@@ -142,6 +145,7 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
             keep = min(max(nb_WP + block, nb_WP), max_WP - m)
             Vb[:keep] = X_all[:keep]
             HV[:keep] = HX_all[:keep]
+            Hp[:keep, :keep] = vec[:, :keep].T @ Hp[:ndim, :ndim] @ vec[:, :keep]
             ndim = keep
             W = orthonormalize(new, ndim)
             if W is None:
@@ -149,13 +153,20 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
             m = W.shape[0]
         Vb[ndim:ndim + m] = W
         HV[ndim:ndim + m] = apply(W, phase_prec)
+        # projected matrix: only the columns of the new vectors are computed,
+        # Hp[i, j] = <Vb_i | H Vb_j> for i < j mirrored (H symmetric), the new
+        # diagonal block symmetrised
+        G = _gram(Vb[:ndim + m], HV[ndim:ndim + m]).cpu().numpy()
+        Hp[:ndim + m, ndim:ndim + m] = G
+        Hp[ndim:ndim + m, :ndim] = G[:ndim].T
+        Hp[ndim:ndim + m, ndim:ndim + m] = 0.5 * (G[ndim:] + G[ndim:].T)
         ndim += m
-        Hp = _gram(Vb[:ndim], HV[:ndim])
-        Hp = 0.5 * (Hp + Hp.T)
-        ev, vec = np.linalg.eigh(Hp.cpu().numpy())
-        vec_t = torch.tensor(vec, device=dev)
-        X_all = vec_t.T @ Vb[:ndim]
-        HX_all = vec_t.T @ HV[:ndim]
+        ev, vec = np.linalg.eigh(Hp[:ndim, :ndim])
+        # Ritz vectors: the nb_WP wanted ones and those a restart keeps
+        nk = min(ndim, nb_WP + block)
+        vec_t = torch.tensor(np.ascontiguousarray(vec[:, :nk].T), device=dev)
+        X_all = vec_t @ Vb[:ndim]
+        HX_all = vec_t @ HV[:ndim]
         E = ev[:nb_WP]
         X = X_all[:nb_WP]
         R = HX_all[:nb_WP] - torch.tensor(E, device=dev)[:, None] * X
@@ -184,11 +195,12 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
             HV[:keep] = apply(Vb[:keep].contiguous(), PREC_FP64)
             ndim = keep
             phase_prec = PREC_FP64
-            Hp = _gram(Vb[:ndim], HV[:ndim])
-            ev, vec = np.linalg.eigh(0.5 * (Hp + Hp.T).cpu().numpy())
-            vec_t = torch.tensor(vec, device=dev)
-            X_all = vec_t.T @ Vb[:ndim]
-            HX_all = vec_t.T @ HV[:ndim]
+            G = _gram(Vb[:ndim], HV[:ndim]).cpu().numpy()
+            Hp[:ndim, :ndim] = 0.5 * (G + G.T)
+            ev, vec = np.linalg.eigh(Hp[:ndim, :ndim])
+            vec_t = torch.tensor(np.ascontiguousarray(vec.T), device=dev)
+            X_all = vec_t @ Vb[:ndim]
+            HX_all = vec_t @ HV[:ndim]
             E = ev[:nb_WP]
             X = X_all[:nb_WP]
             R = HX_all[:nb_WP] - torch.tensor(E, device=dev)[:, None] * X

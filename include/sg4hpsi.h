@@ -187,14 +187,19 @@ int sg4_plan_set_jacobi_triatomic(sg4_plan *p, const double *prm);
  * kernels sum into their outputs with hardware floating-point atomics, which
  * are exact on that memory only.  Host, pinned, managed, fine-grained or
  * uncached memory, and memory of another GPU, is rejected with SG4_ERR_ARG
- * before anything is enqueued. */
+ * before anything is enqueued.
+ * The block is staged in the rows layout padded with zero vectors to the next
+ * power of two (nvec <= 16) or multiple of 16 within max_nvec: the kernels
+ * carry 2^g <= 16 vectors per work item, and one wider item per term costs
+ * less than several narrow ones. */
 int sg4_apply(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);
 /* Allocation-free variant of sg4_apply, for callers that cannot allocate
  * device memory (code under HIP graph capture, serving loops, harnesses that
  * forbid hipMalloc).  No counterpart in the reference.  It calls no hipMalloc,
  * hipFree or allocating library routine: the two rows-layout staging blocks
  * live in a file-scope device array of 256 MiB per GPU (held until the
- * process ends), so nb_ba x nvec <= 16777216 (larger blocks: SG4_ERR_ARG),
+ * process ends), so nb_ba x W <= 16777216, W being nvec padded as in
+ * sg4_apply (larger blocks: SG4_ERR_ARG),
  * and it only uses work items, FP32 copies and matrix-core tables that an
  * earlier sg4_apply with the same nvec, precision and launch configuration
  * has prepared (otherwise SG4_ERR_STATE).  Every call on a GPU shares that
@@ -311,7 +316,10 @@ int sg4_grad_V(sg4_plan *p, int32_t nvec, const double *phi, const double *psi, 
  * temporary blocks.  The reference's only tuner scans OpenMP thread counts
  * (Tune_SG4threads_HPsi, SRC/sub_main/vib.f90:2013-2123). */
 int sg4_plan_autotune(sg4_plan *p, int32_t nvec, int32_t prec, float *best_ms);
-/* the launch parameters in use for (nvec, prec): threads, LDS budget, grid, xcd */
+/* the launch parameters in use for (nvec, prec): threads, LDS budget, grid, xcd.
+ * A block width with no parameters of its own (tuned or set) uses those of the
+ * nearest configured width of the same precision (the narrowest wider one
+ * first), else the defaults. */
 int sg4_plan_config(const sg4_plan *p, int32_t nvec, int32_t prec, int32_t *cfg4);
 /* set the launch parameters for (nvec, prec) (e.g. a configuration saved from
  * an earlier autotune): threads 256 or 512 (768 for the FP32 mode on the split

@@ -94,6 +94,33 @@ def test_f32x2_batch_of_uneven_instances(cuda):
         assert e2 < TOL2 and rc < KCOND
 
 
+@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_F32X2])
+def test_narrow_blocks_take_the_launch_set_for_16(cuda, prec):
+    """With only the 16-vector launch set (the pinned launches of the 10D
+    workload), blocks of 1..16 vectors run with it (sg4_apply pads them to 2,
+    4, 8 or 16 vectors; sg4_plan_config reports the launch) and keep the
+    parity of the mode: 1e-12 in FP64, the split-precision bound otherwise."""
+    D, L = 6, 6
+    b = E.SG4Basis(D, L)
+    op = E.SG4Operator(b, max_nvec=16, potential=("henon_heiles", 0.111803))
+    pinned = {E.PREC_FP64: (512, 163840, 2048, 3), E.PREC_F32X2: (1024, 163840, 512, 2)}[prec]
+    op.set_config(16, prec, *pinned)
+    o = O.SG4(D, L)
+    V = o.grid_hh(0.111803)
+    rng = np.random.default_rng(21)
+    for nvec in (1, 3, 5, 9, 16):
+        c = op.config(nvec, prec)
+        assert (c["nthreads"], c["lds_budget"], c["grid"], c["xcd"]) == pinned
+        psi = rng.standard_normal((b.nb_ba, nvec))
+        got, ref = op.apply_host(psi, prec), o.hpsi(psi, V, -0.5 * np.ones(D))
+        e2, em = errs(got, ref)
+        print(f"prec {prec} nvec {nvec}: 2-norm {e2:.2e} max {em:.2e}")
+        if prec == E.PREC_FP64:
+            assert em < 1e-12
+        else:
+            assert e2 < TOL2 and cond_ratio(got, ref, o, psi, V, D) < KCOND
+
+
 def test_f32x2_10d_L8_block16_against_fp64(cuda):
     """The headline workload: every vector of the 16-block against the FP64
     apply (itself at 1e-12 of the oracle) in the 2-norm, two of them against

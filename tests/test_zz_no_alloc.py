@@ -39,9 +39,10 @@ def test_noalloc_equals_apply(cuda, prec):
     e16, e5 = rel(Y16, R16), rel(Y5, R5)
     print(f"noalloc vs apply: {e16:.2e} (16 vectors), {e5:.2e} (5 vectors)")
     assert e16 < 1e-12 and e5 < 1e-12
-    # a block size the plan was never prepared for is refused, not built
-    X7 = torch.randn(7, b.nb_ba, device=cuda, dtype=torch.float64, generator=g)
-    assert call_noalloc(op, X7, torch.empty_like(X7), prec) == -5   # SG4_ERR_STATE
+    # a block whose padded width (3 -> 4 vectors) the plan was never prepared
+    # for is refused, not built (16 and 5 -> 8 were prepared above)
+    X3 = torch.randn(3, b.nb_ba, device=cuda, dtype=torch.float64, generator=g)
+    assert call_noalloc(op, X3, torch.empty_like(X3), prec) == -5   # SG4_ERR_STATE
 
 
 def test_noalloc_refuses_block_beyond_scratch(cuda):
