@@ -26,6 +26,7 @@ EXPORTS = [
     "sg4_apply_graph", "sg4_plan_set_option", "sg4_plan_split_info", "sg4_plan_mfma_stats",
     "sg4_plan_destroy", "sg4_apply_noalloc", "sg4_cheby_propagate", "sg4_cheby_field_gradient",
     "sg4_plan_grid_dipole", "sg4_plan_set_operator_type1", "sg4_taylor_propagate",
+    "sg4_block_gram_work", "sg4_block_gram", "sg4_block_combine",
 ]
 
 
@@ -113,6 +114,9 @@ def lib():
                                                C.POINTER(C.c_double), D_P]
         L.sg4_taylor_propagate.argtypes = [vp, vp, i32, vp, C.POINTER(TaylorParams),
                                            C.POINTER(i32)]
+        L.sg4_block_gram_work.argtypes = [i32, i32, C.POINTER(i64)]
+        L.sg4_block_gram.argtypes = [vp, i32, i32, i64, vp, i64, vp, i64, vp, vp, i64]
+        L.sg4_block_combine.argtypes = [vp, i32, i32, i64, vp, vp, i64, C.c_double, vp, i64]
         _lib = L
     return _lib
 

@@ -25,7 +25,7 @@ def hipcc():
 
 
 SOURCES = ("sg4_setup.cpp", "sg4_hpsi.hip", "sg4_k_split_f64.hip", "sg4_k_split_f32.hip", "sg4_k_mfma.hip", "sg4_cheby.hip",
-           "sg4_k_grid_f64.hip", "sg4_k_grid_f32.hip")
+           "sg4_k_grid_f64.hip", "sg4_k_grid_f32.hip", "sg4_block.hip")
 
 
 def build(verbose=False):

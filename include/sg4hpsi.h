@@ -299,6 +299,28 @@ typedef struct {
 int sg4_taylor_propagate(sg4_plan *H, const double *mu, int32_t ns, double *psi,
                          const sg4_taylor_params *prm, int32_t *orders);
 
+/* ---- dense products of the device block Davidson ------------------------
+ * The projections, Gram matrices and rotations of sub_module_Davidson.f90
+ * (Rayleigh-Ritz :59-763, double Gram-Schmidt :1529-1547), which the
+ * reference computes with dot_product / matmul over the packed vectors, for
+ * blocks of a few to ~100 rows of length n = nb_ba.  Rows are contiguous
+ * (row i of A at A + i lda), all pointers DEVICE memory as for sg4_apply;
+ * the kernels run on the plan's stream (the plan only provides stream and
+ * device).  No counterpart in the reference as entry points.
+ * sg4_block_gram: C (m x q, row-major) = A B^T, m, q <= 124 with
+ *   ceil(m/4) ceil(q/4) <= 512; work: nwork doubles of scratch (at least
+ *   m q; sg4_block_gram_work gives the size that lets every workgroup keep
+ *   its partial, 512 m q); partial sums are added in a fixed order, so the
+ *   result is reproducible bit for bit.
+ * sg4_block_combine: Y (k x n) = beta Y + U X, U (k x d, row-major, DEVICE),
+ *   X d rows, k d <= 8192; Y must not overlap X; with beta = 0, Y is not
+ *   read. */
+int sg4_block_gram_work(int32_t m, int32_t q, int64_t *nwork);
+int sg4_block_gram(sg4_plan *p, int32_t m, int32_t q, int64_t n, const double *A, int64_t lda,
+                   const double *B, int64_t ldb, double *C, double *work, int64_t nwork);
+int sg4_block_combine(sg4_plan *p, int32_t k, int32_t d, int64_t n, const double *U,
+                      const double *X, int64_t ldx, double beta, double *Y, int64_t ldy);
+
 /* Derivative of <phi_v|H|psi_v> with respect to every grid value of the
  * potential: grad[(p)*nvec + v] = W_t w(p) (B_t phi_v)(p) (B_t psi_v)(p) for the
  * grid point p of term t (w(p): product of the 1-D quadrature weights).  With
