@@ -154,7 +154,8 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
     max_WP = max_WP or max(4 * nb_WP, nb_WP + 4 * block)
     max_WP = min(max_WP, nb)
     dev = torch.device(device)
-    E0 = ho_zeroth_order_diagonal(op.basis, dev)
+    E0 = (op.zeroth_order_diagonal(dev) if hasattr(op, "zeroth_order_diagonal")
+          else ho_zeroth_order_diagonal(op.basis, dev))
     Vb = torch.zeros(max_WP, nb, device=dev, dtype=torch.float64)   # basis (rows = vectors)
     HV = torch.zeros_like(Vb)
     Hp = np.zeros((max_WP, max_WP))                                    # <Vb_i | H Vb_j>

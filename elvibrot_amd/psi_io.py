@@ -34,8 +34,11 @@ the readers take the forms list-directed input allows (blanks or commas,
 ``D`` exponents, repeat counts ``r*c``, values over several records, a
 complex constant split over two records, namelist items ``NAME= value``),
 with each READ starting on a new record as in Fortran.
-Only one electronic state and no extra index (``i_h = i_e = 1``) occur for
-the SG4 bases here.
+Several electronic states (``nb_e`` > 1, the multi-surface operators of
+elvibrot_amd.surfaces) put the nb_e components one after the other, nb_ba
+rows each, with ``i_e`` their 1-based state index, as the reference's loops
+over i_e, i_h, i_b do (Write_Psi_nDBasis :1525-1530); there is no extra
+index (``i_h = 1``).
 """
 import re
 import struct
@@ -130,16 +133,27 @@ def _as_complex(vals, n):
     return r[0::2] + 1j * r[1::2], 2 * n
 
 
-def write_psi(path, psi, basis, version=0, epsi=0.0, T=0.0, formatted=True):
-    """Write the vectors ``psi`` (nb_ba x n, real or complex) as a wave-packet
-    file (sub_save_psi, SRC/sub_WP/sub_module_psi_io.f90:65-124)."""
+def _index_list(basis, nb_e):
+    """(nb_e nb_ba, D + 2): i_1 .. i_D, i_h = 1, i_e of every row."""
+    idx = packed_indices(basis) + 1
+    nb = idx.shape[0]
+    out = np.ones((nb_e * nb, idx.shape[1] + 2), dtype=np.int64)
+    for e in range(nb_e):
+        out[e * nb:(e + 1) * nb, :-2] = idx
+        out[e * nb:(e + 1) * nb, -1] = e + 1
+    return out
+
+
+def write_psi(path, psi, basis, version=0, epsi=0.0, T=0.0, formatted=True, nb_e=1):
+    """Write the vectors ``psi`` (nb_e nb_ba x n, real or complex) as a
+    wave-packet file (sub_save_psi, SRC/sub_WP/sub_module_psi_io.f90:65-124)."""
     psi = np.asarray(psi)
     if psi.ndim == 1:
         psi = psi[:, None]
     nb, n = psi.shape
-    if nb != basis.nb_ba:
-        raise ValueError("psi rows do not match the basis")
-    idx = packed_indices(basis) + 1
+    if nb != nb_e * basis.nb_ba:
+        raise ValueError("psi rows do not match the basis and the number of states")
+    full = _index_list(basis, nb_e)
     cplx = np.iscomplexobj(psi)
     if not formatted:
         if version != 1:
@@ -152,10 +166,9 @@ def write_psi(path, psi, basis, version=0, epsi=0.0, T=0.0, formatted=True):
             rec(struct.pack("<i", nb))
             rec(struct.pack("<i", 1))
             rec(b"X")
-            ones = np.ones((nb, 2), dtype=np.int32)
-            full = np.ascontiguousarray(np.hstack([idx, ones]).astype("<i4"))
+            fi = np.ascontiguousarray(full.astype("<i4"))
             for r in range(nb):
-                rec(full[r].tobytes())
+                rec(fi[r].tobytes())
             for j in range(n):
                 v = psi[:, j].astype("<c16" if cplx else "<f8")
                 rec(struct.pack("<d", T) + v.tobytes())
@@ -167,16 +180,16 @@ def write_psi(path, psi, basis, version=0, epsi=0.0, T=0.0, formatted=True):
                 v = psi[:, j]
                 keep = np.abs(v) >= epsi
                 for r in np.nonzero(keep)[0]:
-                    ind = " ".join(str(int(i)) for i in idx[r])
+                    ind = " ".join(str(int(i)) for i in full[r])
                     if cplx:
-                        f.write(f" {ind} 1 1 {_fmt_real(v[r].real)} {_fmt_real(v[r].imag)}\n")
+                        f.write(f" {ind} {_fmt_real(v[r].real)} {_fmt_real(v[r].imag)}\n")
                     else:
-                        f.write(f" {ind} 1 1 {_fmt_real(v[r])}\n")
+                        f.write(f" {ind} {_fmt_real(v[r])}\n")
                 f.write(f" end wp {j + 1}\n")
         elif version == 1:
             f.write(f"&HEADERFILE\n VERSION_FILE=1,\n NB_PSI={n},\n NB_TOT={nb},\n /\n")
             for r in range(nb):
-                f.write(" " + " ".join(str(int(i)) for i in idx[r]) + " 1 1\n")
+                f.write(" " + " ".join(str(int(i)) for i in full[r]) + "\n")
             for j in range(n):
                 v = psi[:, j]
                 if cplx:
@@ -191,17 +204,19 @@ def write_psi(path, psi, basis, version=0, epsi=0.0, T=0.0, formatted=True):
             raise ValueError("version must be 0 or 1")
 
 
-def _rows_of(basis):
-    idx = packed_indices(basis) + 1
-    return {tuple(int(i) for i in r): n for n, r in enumerate(idx)}
+def _rows_of(basis, nb_e=1):
+    """(i_1 .. i_D, i_e) -> row of the nb_e-state packed vector"""
+    full = _index_list(basis, nb_e)
+    return {tuple(int(i) for i in r[:-2]) + (int(r[-1]),): n for n, r in enumerate(full)}
 
 
-def read_psi(path, basis, cplx=False, formatted=True):
-    """Read a wave-packet file into an (nb_ba, n) array (complex when
+def read_psi(path, basis, cplx=False, formatted=True, nb_e=1):
+    """Read a wave-packet file into an (nb_e nb_ba, n) array (complex when
     ``cplx``).  Returns (psi, info) with info = dict(version, nb_psi, nb_tot,
-    T (list, version 1), skipped (functions absent from the basis))."""
+    T (list, version 1), skipped (functions absent from the basis or of a
+    state beyond nb_e))."""
     D = basis.D
-    nbr = basis.nb_ba
+    nbr = nb_e * basis.nb_ba
     dt = np.complex128 if cplx else np.float64
     info = dict(skipped=0, T=[])
     if not formatted:
@@ -218,8 +233,9 @@ def read_psi(path, basis, cplx=False, formatted=True):
         nb = struct.unpack("<i", recs[2])[0]
         info.update(version=struct.unpack("<i", recs[3])[0], nb_psi=n, nb_tot=nb)
         lst = [np.frombuffer(recs[5 + r], dtype="<i4") for r in range(nb)]
-        rows = _rows_of(basis)
-        target = np.array([rows.get(tuple(int(i) for i in x[:D]), -1) for x in lst])
+        rows = _rows_of(basis, nb_e)
+        target = np.array([rows.get(tuple(int(i) for i in x[:D]) + (int(x[D + 1]),), -1)
+                           for x in lst])
         psi = np.zeros((nbr, n), dtype=dt)
         for j in range(n):
             b = recs[5 + nb + j]
@@ -232,7 +248,7 @@ def read_psi(path, basis, cplx=False, formatted=True):
     with open(path) as f:
         lines = f.read().splitlines()
     first = lines[0].split()
-    rows = _rows_of(basis)
+    rows = _rows_of(basis, nb_e)
     try:
         ints = [int(x) for x in first]
     except ValueError:
@@ -252,7 +268,7 @@ def read_psi(path, basis, cplx=False, formatted=True):
             tok = line.split()
             if not tok:
                 continue
-            ind = tuple(int(x) for x in tok[:D])
+            ind = tuple(int(x) for x in tok[:D]) + (int(tok[D + 1]),)
             vals = [float(x.replace("D", "E").replace("d", "e")) for x in tok[D + 2:]]
             r = rows.get(ind)
             if r is None:
@@ -270,7 +286,7 @@ def read_psi(path, basis, cplx=False, formatted=True):
     lst = []
     for r in range(nb):
         v, pos = _ld_read(lines, pos, D + 2, "index")
-        lst.append(tuple(int(x.real) for x in v[:D]))
+        lst.append(tuple(int(x.real) for x in v[:D]) + (int(v[D + 1].real),))
     target = np.array([rows.get(t, -1) for t in lst])
     psi = np.zeros((nbr, n), dtype=dt)
     ok = target >= 0

@@ -354,3 +354,42 @@ def test_restart_file_written_by_fortran_runtime_is_read(tmp_path, fortran_reade
     assert r.returncode == 0, r.stdout + r.stderr
     T, got = psi_io.read_restart(str(f), 3, 11)
     assert T == 7.5 and np.array_equal(got, psis)
+
+
+@pytest.mark.parametrize("version,formatted", [(0, True), (1, True), (1, False)])
+def test_two_state_files_round_trip(tmp_path, version, formatted):
+    """nb_e = 2: the components one after the other, i_e the state index
+    (the reference writer's i_e, i_h, i_b loop order), back bit for bit; a
+    one-state read keeps the first state and skips the second's rows."""
+    b = E.SG4Basis(3, 3)
+    nb = b.nb_ba
+    rng = np.random.default_rng(21)
+    psi = rng.standard_normal((2 * nb, 2)) + 1j * rng.standard_normal((2 * nb, 2))
+    p = tmp_path / f"wp2_{version}_{formatted}"
+    psi_io.write_psi(str(p), psi, b, version=version, formatted=formatted, nb_e=2)
+    got, info = psi_io.read_psi(str(p), b, cplx=True, formatted=formatted, nb_e=2)
+    assert np.array_equal(got, psi) and info["skipped"] == 0 and info["nb_tot"] == 2 * nb
+    one, info1 = psi_io.read_psi(str(p), b, cplx=True, formatted=formatted)
+    assert np.array_equal(one, psi[:nb])
+    assert info1["skipped"] == 2 * nb            # the second state's rows of both vectors
+    if formatted and version == 0:
+        rows = [l.split() for l in open(p).read().splitlines()[1:] if "end" not in l]
+        ie = np.array([int(r[b.D + 1]) for r in rows[:2 * nb]])
+        assert np.array_equal(ie, np.repeat([1, 2], nb))
+    with pytest.raises(ValueError):
+        psi_io.write_psi(str(p), psi[:nb + 1], b, nb_e=2)
+
+
+def test_two_state_version1_read_by_fortran_runtime(tmp_path, fortran_reader):
+    """A two-state version-1 file read by amdflang's runtime: nb_tot = 2 nb_ba
+    index lines with i_e = 1 then 2, and every coefficient bit for bit."""
+    b = E.SG4Basis(3, 3)
+    nb = b.nb_ba
+    psi = np.random.default_rng(22).standard_normal((2 * nb, 2))
+    p = tmp_path / "wp2"
+    psi_io.write_psi(str(p), psi, b, version=1, T=1.5, nb_e=2)
+    nb_psi, nb_tot, version, ind, T, v = _run_reader(fortran_reader, "v1f", 3, False, p, tmp_path)
+    assert (nb_psi, nb_tot, version) == (2, 2 * nb, 1)
+    assert np.array_equal(ind[:, 3], np.ones(2 * nb)) and np.array_equal(ind[:, 4], np.repeat([1, 2], nb))
+    assert np.array_equal(ind[:nb, :3], ind[nb:, :3])
+    assert np.all(T == 1.5) and np.array_equal(v, psi.T)
