@@ -18,3 +18,8 @@ sed -n '80,87p' "$H/dat_Davidson"              > "$OUT/psi_v0_3d_dat_Davidson.tx
 sed -n '67,70p' "$H/dat_Davidson_1D_WithGrid"  > "$OUT/psi_v0_1d_dat_Davidson_1D_WithGrid.txt"
 sed -n '68,77p' "$H/dat_Davidson_WithGrid"     > "$OUT/psi_v0_3d_dat_Davidson_WithGrid.txt"
 sed -n '65,68p' "$H/dat_Davidson_AutoContract" > "$OUT/psi_v0_2d_dat_Davidson_AutoContract.txt"
+# 12D pyrazine two-surface propagation at L=1 (run_jobs: shell_run 3 1):
+# autocorrelation T(au), Re C, Im C, |C| every 0.1 fs up to 6 fs
+cp "$REF/Working_tests/MPI_tests/12D_propagation_S3/benchmark" "$OUT/pyrazine12d_L1_autocorr_benchmark.txt"
+# 24D pyrazine at L=2 (run_jobs: shell_run 3 2; the S0 variant stores the same values)
+cp "$REF/Working_tests/MPI_tests/24D_propagation_S3/benchmark" "$OUT/pyrazine24d_L2_autocorr_benchmark.txt"
