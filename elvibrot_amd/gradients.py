@@ -55,9 +55,28 @@ def lowest_eigenpairs(H, n):
     return E, R, Lsel
 
 
-def eigenvalue_gradients(op, dV_list, n, device="cuda:0"):
-    """dE_k/dtheta_j for the n lowest levels; dV_list[j] is the grid dV/dtheta_j."""
-    H = dense_matrix(op)
-    E, R, Lv = lowest_eigenpairs(H, n)
+def eigenvalue_gradients(op, dV_list, n, device="cuda:0", method=None, conv_resi=1e-9):
+    """dE_k/dtheta_j for the n lowest levels; dV_list[j] is the grid dV/dtheta_j.
+
+    method "davidson": eigenpairs from the device block Davidson (symmetric
+    operators, any basis size: no dense matrix); "dense": the host matrix
+    with left and right eigenvectors (any operator, small bases: O(nb_ba^3)).
+    Default: "davidson" for the constant-coefficient type-1 operators (H
+    symmetric), "dense" otherwise (type 10 on a finite grid is not)."""
+    if method is None:
+        method = "davidson" if getattr(op, "type_op", 1) == 1 else "dense"
+    if method == "davidson":
+        from .davidson import sub_propagation_Davidson
+        r = sub_propagation_Davidson(op, n, max_it=500, conv_resi=conv_resi, conv_ene=1e-12,
+                                     device=device)
+        if not r.converged.all():
+            raise RuntimeError("Davidson did not converge the requested levels")
+        E = np.asarray(r.E)
+        R = np.ascontiguousarray(r.X.cpu().numpy().T)
+        Lv = R
+    elif method == "dense":
+        E, R, Lv = lowest_eigenpairs(dense_matrix(op), n)
+    else:
+        raise ValueError("method must be 'davidson' or 'dense'")
     g = grad_V(op, Lv, R, device)          # (sum_nq, n)
     return E, np.stack([g.T @ dV for dV in dV_list], axis=1), g

@@ -55,7 +55,7 @@ def test_henon_heiles_dE_dlambda_vs_finite_difference(cuda):
     dV = (q[:, :-1] ** 2 * q[:, 1:] - q[:, 1:] ** 3 / 3).sum(1)
     Vl = lambda x: 0.5 * (q ** 2).sum(1) + x * dV
     op = E.SG4Operator(b, max_nvec=8, potential=Vl(lam), ck=-0.5 * np.ones(D))
-    Ev, dE, _ = eigenvalue_gradients(op, [dV], n)
+    Ev, dE, _ = eigenvalue_gradients(op, [dV], n, method="dense")
     fd = []
     for s in (+1, -1):
         ops = E.SG4Operator(b, max_nvec=8, potential=Vl(lam + s * h), ck=-0.5 * np.ones(D))
@@ -170,3 +170,35 @@ def test_jacobi_L10_gradients_hellmann_feynman_and_finite_differences(cuda):
         fd = (ev[0] - ev[1]) / (2 * h)
         print(k, "fd", fd, f"({time.time() - t0:.1f} s)", flush=True)
         assert np.max(np.abs(dE[:, j] - fd) / np.maximum(np.abs(fd), 1e-4)) < 1e-6
+
+
+def test_davidson_gradients_equal_dense_and_scale(cuda):
+    """Eigenvalue gradients from the device Davidson's eigenvectors (no dense
+    matrix) equal the dense path's at 5D L=5 (1e-8), and at 8D L=6 (nb_ba
+    beyond a dense host diagonalisation) the ground state's dE/dlambda agrees
+    with central differences of Davidson ground energies (1e-6 relative)."""
+    def hh(D, L, lam):
+        b = E.SG4Basis(D, L)
+        q = np.concatenate([b.term_grid(t) for t in range(b.nterms)])
+        dV = (q[:, :-1] ** 2 * q[:, 1:] - q[:, 1:] ** 3 / 3).sum(1)
+        V = lambda x: 0.5 * (q ** 2).sum(1) + x * dV
+        return b, dV, V
+    lam = 0.111803
+    b, dV, V = hh(5, 5, lam)
+    op = E.SG4Operator(b, max_nvec=16, potential=V(lam), ck=-0.5 * np.ones(5))
+    Ed, gd, _ = eigenvalue_gradients(op, [dV], 3, method="dense")
+    Ev, gv, _ = eigenvalue_gradients(op, [dV], 3, method="davidson")
+    print("5D L=5:", Ev - Ed, gv[:, 0] - gd[:, 0])
+    assert np.max(np.abs(Ev - Ed)) < 1e-9 and np.max(np.abs(gv - gd)) < 1e-8
+    from elvibrot_amd.davidson import sub_propagation_Davidson
+    b, dV, V = hh(8, 6, lam)
+    print(f"8D L=6: nb_ba {b.nb_ba}", flush=True)
+    op = E.SG4Operator(b, max_nvec=16, potential=V(lam), ck=-0.5 * np.ones(8))
+    Ev, gv, _ = eigenvalue_gradients(op, [dV], 1)
+    h, e = 1e-4, []       # central-difference truncation ~ h^2 |d3E/dlambda3| / 6 ~ 2e-8
+    for s in (+1, -1):
+        o2 = E.SG4Operator(b, max_nvec=16, potential=V(lam + s * h), ck=-0.5 * np.ones(8))
+        e.append(sub_propagation_Davidson(o2, 1, max_it=500, conv_resi=1e-9, conv_ene=1e-13).E[0])
+    fd = (e[0] - e[1]) / (2 * h)
+    print(f"dE0/dlambda {gv[0, 0]:.10e} fd {fd:.10e}")
+    assert abs(gv[0, 0] - fd) < 1e-6 * abs(fd)
