@@ -54,6 +54,26 @@ def test_fp64_parity_device_entries(cuda):
     assert rel(yr.cpu().numpy(), ref) < TOL
 
 
+@pytest.mark.parametrize("max_nvec,nvec", [(12, 9), (12, 12), (32, 20), (16, 5), (4, 3), (40, 33)])
+def test_padded_block_widths(cuda, max_nvec, nvec):
+    """sg4_apply stages a block padded with zero vectors to the next power of
+    two (<= 16) or multiple of 16 when max_nvec allows (12 -> 16 does not:
+    unpadded), and returns exactly the nvec columns asked for."""
+    import torch
+    D, L = 5, 4
+    b = E.SG4Basis(D, L)
+    op = E.SG4Operator(b, max_nvec=max_nvec)
+    o = O.SG4(D, L)
+    psi = np.random.default_rng(max_nvec * 100 + nvec).standard_normal((b.nb_ba, nvec))
+    ref = o.hpsi(psi, o.grid_hh(0.111803), -0.5 * np.ones(D))
+    x = torch.tensor(psi.T.copy(), device=cuda)
+    ybuf = torch.full((nvec + 1, b.nb_ba), 7.0, device=cuda, dtype=torch.float64)
+    op.apply(x, ybuf[:nvec])
+    torch.cuda.synchronize()
+    assert rel(ybuf[:nvec].cpu().numpy().T, ref) < TOL
+    assert torch.all(ybuf[nvec] == 7.0)          # nothing written past the block
+
+
 def test_fp64_host_potential_equals_device_potential(cuda):
     D, L = 5, 4
     b = E.SG4Basis(D, L)
