@@ -15,6 +15,7 @@
 !                               (Tab_nDval, WeightSG, tab_iB_OF_SRep_TO_iB
 !                               with 0 = absent, tab_basisPrimSG(l,k) matrices)
 !      sg4_gpu_set_op_type1 / sg4_gpu_set_op_type10 -- operator grids
+!      sg4_gpu_set_op_type1_nb0 -- grids of several electronic surfaces
 !      sg4_gpu_TabOpPsi      -- block H.psi on host arrays PsiB(nb_ba,nvec)
 !      sg4_gpu_TabOpPsi_dev  -- block H.psi on DEVICE blocks (c_ptr)
 !  * sub_TabOpPsi_FOR_SGtype4 / sub_OpPsi_FOR_SGtype4 over small param_psi /
@@ -78,6 +79,10 @@ MODULE mod_sg4_gpu
   TYPE, PUBLIC :: param_Op
     type(c_ptr)        :: setup = c_null_ptr
     type(c_ptr)        :: plan  = c_null_ptr
+    ! several electronic surfaces (nb0 = nb_bie > 1): one plan per block
+    ! (ib0, jb0) of GridOp(:,ib0,jb0,:), c_null_ptr for a zero block
+    integer            :: nb0 = 1
+    type(c_ptr), allocatable :: blocks(:,:)
     integer            :: nb_ba = 0
     integer            :: type_Op = 1
     integer            :: D = 0
@@ -157,7 +162,7 @@ MODULE mod_sg4_gpu
 
   PUBLIC :: sub_TabOpPsi_FOR_SGtype4, sub_OpPsi_FOR_SGtype4
   PUBLIC :: init_para_Op_SG4_HenonHeiles, dealloc_para_Op_SG4, alloc_psi
-  PUBLIC :: sg4_gpu_init_BasisnD, sg4_gpu_set_op_type1, sg4_gpu_set_op_type10
+  PUBLIC :: sg4_gpu_init_BasisnD, sg4_gpu_set_op_type1, sg4_gpu_set_op_type10, sg4_gpu_set_op_type1_nb0
   PUBLIC :: sg4_gpu_TabOpPsi, sg4_gpu_TabOpPsi_dev
 
 CONTAINS
@@ -223,14 +228,15 @@ CONTAINS
   !! reads them (sub_module_basis_BtoG_GtoB_SG4.f90:1005-1027): entries
   !! outside 1..nb_ba are absent and terms with |WeightSG| < 1e-6 are
   !! skipped (their weight is set to zero).
-  SUBROUTINE sg4_gpu_init_BasisnD(para_Op, Tab_nDval, WeightSG, tab_iB, prim, nb_ba, max_nvec)
+  SUBROUTINE sg4_gpu_init_BasisnD(para_Op, Tab_nDval, WeightSG, tab_iB, prim, nb_ba, max_nvec, nb0)
     TYPE(param_Op), intent(inout) :: para_Op
     integer, intent(in) :: Tab_nDval(:,:)
     real(c_double), intent(in) :: WeightSG(:)
     integer, intent(in) :: tab_iB(:)
     TYPE(sg4_prim_basis), intent(in) :: prim(0:,:)
     integer, intent(in) :: nb_ba, max_nvec
-    integer :: D, L1, Lmax, nterms, k, l, t, nq, nb
+    integer, intent(in), optional :: nb0
+    integer :: D, L1, Lmax, nterms, k, l, t, nq, nb, ib0, jb0
     integer(c_int64_t) :: sum_nb, sum_nq, nbt, nqt, off, noff
     integer(c_int32_t), allocatable, target :: lev(:,:), map0(:), nqv(:,:), nbv(:,:)
     real(c_double), allocatable, target :: w(:), mats(:), nodes(:), qw(:)
@@ -330,6 +336,23 @@ CONTAINS
     ! the plan copies every table: the local arrays may go afterwards
     CALL sg4_check(sg4_plan_create(tabs, int(max_nvec, c_int32_t), para_Op%plan), &
                    'sg4_gpu_init_BasisnD')
+    para_Op%nb0 = 1
+    IF (present(nb0)) para_Op%nb0 = nb0
+    IF (para_Op%nb0 < 1) STOP ' ERROR in sg4_gpu_init_BasisnD: nb0 < 1'
+    IF (para_Op%nb0 > 1) THEN
+      ! one plan per surface block (ib0, jb0); block (1,1) is para_Op%plan
+      allocate(para_Op%blocks(para_Op%nb0, para_Op%nb0))
+      DO jb0 = 1, para_Op%nb0
+        DO ib0 = 1, para_Op%nb0
+          IF (ib0 == 1 .AND. jb0 == 1) THEN
+            para_Op%blocks(1, 1) = para_Op%plan
+          ELSE
+            CALL sg4_check(sg4_plan_create(tabs, int(max_nvec, c_int32_t), para_Op%blocks(ib0, jb0)), &
+                           'sg4_gpu_init_BasisnD')
+          END IF
+        END DO
+      END DO
+    END IF
     para_Op%nb_ba = nb_ba
     para_Op%D = D
     para_Op%sum_nq = sum_nq
@@ -351,6 +374,48 @@ CONTAINS
     CALL sg4_check(sg4_plan_set_operator(para_Op%plan, op), 'sg4_gpu_set_op_type1')
     para_Op%type_Op = 1
   END SUBROUTINE sg4_gpu_set_op_type1
+
+  !> Type-1 operator on nb0 surfaces: GridV(:,ib0,jb0) = GridOp(:,ib0,jb0) of
+  !! the potential term (sub_OpPsi_SG4.f90:1506-1522, every grid point in term
+  !! order), the kinetic coefficients ck(k) = -G_kk/2 on the diagonal blocks
+  !! only.  An off-diagonal block whose grid is zero everywhere is dropped
+  !! (the reference skips grid_zero terms, :1508); block (ib0, jb0) then acts
+  !! as the single-surface operator on its grid.
+  SUBROUTINE sg4_gpu_set_op_type1_nb0(para_Op, GridV, ck)
+    TYPE(param_Op), intent(inout) :: para_Op
+    real(c_double), intent(in), target :: GridV(:,:,:), ck(:)
+    real(c_double), allocatable, target :: Vb(:), c0(:)
+    TYPE(sg4_operator) :: op
+    integer :: ib0, jb0, n0
+    integer(c_int) :: rc
+    n0 = para_Op%nb0
+    IF (n0 < 2) STOP ' ERROR in sg4_gpu_set_op_type1_nb0: the plan was built for one surface'
+    IF (size(GridV, 1, kind=c_int64_t) /= para_Op%sum_nq .OR. size(GridV, 2) /= n0 .OR. &
+        size(GridV, 3) /= n0 .OR. size(ck) /= para_Op%D) &
+      STOP ' ERROR in sg4_gpu_set_op_type1_nb0: GridV(sum_nq,nb0,nb0) and ck(D) expected'
+    allocate(Vb(para_Op%sum_nq), c0(para_Op%D))
+    c0(:) = 0.0_c_double
+    DO jb0 = 1, n0
+      DO ib0 = 1, n0
+        IF (.NOT. c_associated(para_Op%blocks(ib0, jb0))) CYCLE
+        IF (ib0 /= jb0 .AND. all(GridV(:, ib0, jb0) == 0.0_c_double)) THEN
+          rc = sg4_plan_destroy(para_Op%blocks(ib0, jb0))
+          para_Op%blocks(ib0, jb0) = c_null_ptr
+          CYCLE
+        END IF
+        Vb(:) = GridV(:, ib0, jb0)
+        op%type_op = 1
+        op%V = c_loc(Vb)
+        IF (ib0 == jb0) THEN
+          op%ck = c_loc(ck)
+        ELSE
+          op%ck = c_loc(c0)
+        END IF
+        CALL sg4_check(sg4_plan_set_operator(para_Op%blocks(ib0, jb0), op), 'sg4_gpu_set_op_type1_nb0')
+      END DO
+    END DO
+    para_Op%type_Op = 1
+  END SUBROUTINE sg4_gpu_set_op_type1_nb0
 
   !> Type-10 operator grids (get_OpGrid_type10_OF_ONEDP_FOR_SG4,
   !! SRC/sub_Operator/sub_OpPsi_SG4.f90:327): V, Jac, sqRhoOVERJac on every
@@ -383,6 +448,10 @@ CONTAINS
     integer :: nvec, v0, nv
     nvec = size(PsiB, 2)
     IF (nvec == 0) STOP ' ERROR in sub_TabOpPsi_FOR_SGtype4: size(Psi) = 0'
+    IF (para_Op%nb0 > 1) THEN
+      CALL tabOpPsi_surfaces(para_Op, PsiB, OpPsiB)
+      RETURN
+    END IF
     IF (size(PsiB, 1) /= para_Op%nb_ba .OR. any(shape(OpPsiB) /= shape(PsiB))) &
       STOP ' ERROR in sg4_gpu_TabOpPsi: PsiB and OpPsiB must be (nb_ba, nvec)'
     v0 = 0
@@ -393,6 +462,39 @@ CONTAINS
       v0 = v0 + nv
     END DO
   END SUBROUTINE sg4_gpu_TabOpPsi
+
+  !> Several surfaces on host blocks PsiB(nb0 nb_ba, nvec): the nb0
+  !! components one after the other (i_bhe = i_b + nb_ba (i_e - 1),
+  !! SRC/sub_WP/sub_module_psi_io.f90:1525-1530), OpPsi_ib0 = sum_jb0
+  !! H(ib0,jb0) Psi_jb0 with one library call per block and max_nvec columns.
+  SUBROUTINE tabOpPsi_surfaces(para_Op, PsiB, OpPsiB)
+    TYPE(param_Op), intent(inout) :: para_Op
+    real(c_double), intent(in)    :: PsiB(:,:)
+    real(c_double), intent(inout) :: OpPsiB(:,:)
+    real(c_double), allocatable :: x(:,:), y(:,:)
+    integer :: nb, nvec, v0, nv, ib0, jb0
+    nb = para_Op%nb_ba
+    nvec = size(PsiB, 2)
+    IF (size(PsiB, 1) /= para_Op%nb0 * nb .OR. any(shape(OpPsiB) /= shape(PsiB))) &
+      STOP ' ERROR in sg4_gpu_TabOpPsi: PsiB and OpPsiB must be (nb0*nb_ba, nvec)'
+    allocate(x(nb, min(nvec, int(para_Op%max_nvec))), y(nb, min(nvec, int(para_Op%max_nvec))))
+    OpPsiB(:,:) = 0.0_c_double
+    v0 = 0
+    DO WHILE (v0 < nvec)
+      nv = min(nvec - v0, int(para_Op%max_nvec))
+      DO jb0 = 1, para_Op%nb0
+        x(:, 1:nv) = PsiB((jb0-1)*nb+1:jb0*nb, v0+1:v0+nv)
+        DO ib0 = 1, para_Op%nb0
+          IF (.NOT. c_associated(para_Op%blocks(ib0, jb0))) CYCLE
+          CALL sg4_check(sg4_apply_host(para_Op%blocks(ib0, jb0), int(nv, c_int32_t), x, y, &
+                                        para_Op%precision), 'sg4_gpu_TabOpPsi')
+          OpPsiB((ib0-1)*nb+1:ib0*nb, v0+1:v0+nv) = OpPsiB((ib0-1)*nb+1:ib0*nb, v0+1:v0+nv) + y(:, 1:nv)
+        END DO
+      END DO
+      v0 = v0 + nv
+    END DO
+    deallocate(x, y)
+  END SUBROUTINE tabOpPsi_surfaces
 
   !> H.psi on DEVICE blocks: psi_d and hpsi_d point to nb_ba x nvec
   !! column-major FP64 blocks in device memory (hipMalloc, c_loc of a device
@@ -410,6 +512,14 @@ CONTAINS
     integer(c_int) :: rc
     integer :: v0, nv
     IF (nvec < 1) STOP ' ERROR in sub_TabOpPsi_FOR_SGtype4: size(Psi) = 0'
+    IF (para_Op%nb0 > 1) THEN
+      ! the components of a device block are not contiguous per surface
+      IF (present(stat)) THEN
+        stat = int(SG4_ERR_UNSUPPORTED)
+        RETURN
+      END IF
+      STOP ' ERROR in sg4_gpu_TabOpPsi_dev: several surfaces are applied on host arrays (sg4_gpu_TabOpPsi)'
+    END IF
     a = transfer(psi_d, a)
     b = transfer(hpsi_d, b)
     rc = 0
@@ -431,6 +541,17 @@ CONTAINS
   SUBROUTINE dealloc_para_Op_SG4(para_Op)
     TYPE(param_Op), intent(inout) :: para_Op
     integer(c_int) :: rc
+    integer :: ib, jb
+    IF (allocated(para_Op%blocks)) THEN
+      para_Op%blocks(1, 1) = c_null_ptr          ! = para_Op%plan, destroyed below
+      DO jb = 1, size(para_Op%blocks, 2)
+        DO ib = 1, size(para_Op%blocks, 1)
+          IF (c_associated(para_Op%blocks(ib, jb))) rc = sg4_plan_destroy(para_Op%blocks(ib, jb))
+        END DO
+      END DO
+      deallocate(para_Op%blocks)
+    END IF
+    para_Op%nb0 = 1
     IF (c_associated(para_Op%plan))  rc = sg4_plan_destroy(para_Op%plan)
     IF (c_associated(para_Op%setup)) rc = sg4_setup_destroy(para_Op%setup)
     para_Op%plan = c_null_ptr
@@ -448,6 +569,21 @@ CONTAINS
     IF (size(Psi) == 0) STOP ' ERROR in sub_TabOpPsi_FOR_SGtype4: size(Psi) = 0'
     IF (Psi(1)%cplx) STOP ' ERROR in sub_TabOpPsi_FOR_SGtype4: Psi(1) is complex'
     nvec = size(Psi)
+    IF (para_Op%nb0 > 1) THEN
+      ! nb_tot = nb0 nb_ba: the surface blocks on the whole block of vectors
+      allocate(blk(para_Op%nb0 * para_Op%nb_ba, nvec), hblk(para_Op%nb0 * para_Op%nb_ba, nvec))
+      DO itab = 1, nvec
+        blk(:, itab) = Psi(itab)%RvecB(1:para_Op%nb0 * para_Op%nb_ba)
+      END DO
+      CALL tabOpPsi_surfaces(para_Op, blk, hblk)
+      DO itab = 1, nvec
+        IF (.NOT. allocated(OpPsi(itab)%RvecB)) CALL alloc_psi(OpPsi(itab), para_Op%nb0 * para_Op%nb_ba)
+        OpPsi(itab)%cplx = .FALSE.
+        OpPsi(itab)%RvecB(1:para_Op%nb0 * para_Op%nb_ba) = hblk(:, itab)
+      END DO
+      deallocate(blk, hblk)
+      RETURN
+    END IF
     allocate(blk(para_Op%nb_ba, min(nvec, int(para_Op%max_nvec))))
     allocate(hblk(para_Op%nb_ba, min(nvec, int(para_Op%max_nvec))))
     DO itab = 1, nvec

@@ -8,12 +8,16 @@
 !                              d1gg(nq,nq), d2gg(nq,nq), x(nq), wrho(nq)
 !   type 1:  real64 V(sum_nq), ck(D)
 !   type 10: real64 V(sum_nq), G(D,D,sum_nq), Jac(sum_nq), sqRhoOVERJac(sum_nq)
-!   real64 psi(nb_ba, nvec)
+!   type 21 (type 1 on two electronic surfaces): real64 GridV(sum_nq,2,2), ck(D)
+!   real64 psi(nb_ba, nvec)   (type 21: psi(2 nb_ba, nvec))
 ! It builds the device plan with sg4_gpu_init_BasisnD, applies H on host
 ! arrays (sg4_gpu_TabOpPsi) and on device blocks it allocates itself
 ! (sg4_gpu_TabOpPsi_dev), checks that a host array handed to the device
 ! entry is refused, and writes
 !   real64 hpsi_host(nb_ba, nvec), hpsi_dev(nb_ba, nvec); int32 status_refused
+! (type 21: hpsi_host from sg4_gpu_TabOpPsi and, in place of hpsi_dev, the
+! same through sub_TabOpPsi_FOR_SGtype4 on param_psi; status_refused is the
+! device entry's answer to a two-surface plan)
 PROGRAM test_sg4_reflayout
   USE, INTRINSIC :: iso_c_binding
   USE mod_sg4_gpu
@@ -47,7 +51,9 @@ PROGRAM test_sg4_reflayout
   TYPE(sg4_prim_basis), allocatable :: prim(:,:)
   integer(c_int32_t) :: hdr(9), nq, nb
   integer(c_int32_t), allocatable :: tab_nDval(:,:), tab_iB(:)
-  real(c_double), allocatable :: WeightSG(:), V(:), ck(:), G(:,:,:), Jac(:), sqRJ(:)
+  real(c_double), allocatable :: WeightSG(:), V(:), ck(:), G(:,:,:), Jac(:), sqRJ(:), GridV(:,:,:)
+  TYPE(param_psi), allocatable :: tpsi(:), thpsi(:)
+  integer :: nrow, it
   real(c_double), allocatable, target :: psi(:,:), hpsi_host(:,:), hpsi_dev(:,:), host_blk(:,:)
   integer :: D, Lmax, nterms, nb_ba, nvec, type_op, k, l, u, stat_refused
   integer(c_int64_t) :: sum_nb, sum_nq
@@ -73,16 +79,47 @@ PROGRAM test_sg4_reflayout
     END DO
   END DO
   allocate(V(sum_nq))
+  nrow = nb_ba
   IF (type_op == 1) THEN
     allocate(ck(D))
     read(u) V, ck
+  ELSE IF (type_op == 21) THEN
+    allocate(GridV(sum_nq, 2, 2), ck(D))
+    read(u) GridV, ck
+    nrow = 2 * nb_ba
   ELSE
     allocate(G(D, D, sum_nq), Jac(sum_nq), sqRJ(sum_nq))
     read(u) V, G, Jac, sqRJ
   END IF
-  allocate(psi(nb_ba, nvec), hpsi_host(nb_ba, nvec), hpsi_dev(nb_ba, nvec))
+  allocate(psi(nrow, nvec), hpsi_host(nrow, nvec), hpsi_dev(nrow, nvec))
   read(u) psi
   close(u)
+
+  IF (type_op == 21) THEN
+    CALL sg4_gpu_init_BasisnD(para_Op, tab_nDval, WeightSG, tab_iB, prim, nb_ba, int(hdr(9)), nb0=2)
+    CALL sg4_gpu_set_op_type1_nb0(para_Op, GridV, ck)
+    CALL sg4_gpu_TabOpPsi(para_Op, psi, hpsi_host)
+    allocate(tpsi(nvec), thpsi(nvec))
+    DO it = 1, nvec
+      CALL alloc_psi(tpsi(it), nrow)
+      tpsi(it)%RvecB(:) = psi(:, it)
+    END DO
+    CALL sub_TabOpPsi_FOR_SGtype4(tpsi, thpsi, para_Op)
+    DO it = 1, nvec
+      hpsi_dev(:, it) = thpsi(it)%RvecB(1:nrow)
+    END DO
+    nbytes = int(8, c_size_t) * int(nrow, c_size_t) * int(nvec, c_size_t)
+    IF (hipMalloc(psi_d, nbytes) /= 0 .OR. hipMalloc(hpsi_d, nbytes) /= 0) STOP 'hipMalloc failed'
+    CALL sg4_gpu_TabOpPsi_dev(para_Op, nvec, psi_d, hpsi_d, stat=stat_refused)
+    IF (hipFree(psi_d) /= 0 .OR. hipFree(hpsi_d) /= 0) STOP 'hipFree failed'
+    open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+    write(u) hpsi_host, hpsi_dev, int(stat_refused, c_int32_t)
+    close(u)
+    CALL dealloc_para_Op_SG4(para_Op)
+    write(*,*) 'test_sg4_reflayout: two surfaces, nb_ba=', nb_ba, ' nvec=', nvec, &
+               ' device entry status=', stat_refused
+    STOP
+  END IF
 
   CALL sg4_gpu_init_BasisnD(para_Op, tab_nDval, WeightSG, tab_iB, prim, nb_ba, int(hdr(9)))
   IF (type_op == 1) THEN

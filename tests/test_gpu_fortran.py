@@ -39,7 +39,7 @@ def _write_reference_layout(path, o, type_op, grids, psi, max_nvec):
     f64 = lambda a: np.asfortranarray(a, dtype=np.float64).tobytes(order="F")
     i32 = lambda a: np.asarray(a, dtype=np.int32).tobytes(order="F")
     D, L = o.D, o.L
-    nb_ba, nvec = psi.shape
+    nb_ba, nvec = o.nb_ba, psi.shape[1]
     hdr = [D, L, o.nterms, nb_ba, o.sum_nb, o.sum_nq, nvec, type_op, max_nvec]
     tab_iB = o.map.astype(np.int64) + 1                      # -1 (absent) -> 0
     assert tab_iB.min() >= 0 and tab_iB.max() <= nb_ba
@@ -105,3 +105,39 @@ def test_fortran_reference_layout_tables_host_and_device(cuda, tmp_path, case):
     assert np.max(np.abs(host - ref)) / scale < 1e-12
     assert np.max(np.abs(dev - ref)) / scale < 1e-12
     assert refused == -1                                       # SG4_ERR_ARG
+
+
+@pytest.mark.gpu
+def test_fortran_two_surface_operator(cuda, tmp_path):
+    """Two electronic surfaces through the Fortran layer: sg4_gpu_init_BasisnD
+    with nb0 = 2 (one plan per surface block), sg4_gpu_set_op_type1_nb0 with
+    GridV(:,ib0,jb0) and the kinetic part on the diagonal blocks, applied on
+    host arrays (sg4_gpu_TabOpPsi) and through sub_TabOpPsi_FOR_SGtype4 on
+    param_psi vectors of nb_tot = 2 nb_ba (components one after the other):
+    both against the oracle's blocks at 1e-12; the device entry answers a
+    two-surface plan with SG4_ERR_UNSUPPORTED."""
+    assert os.path.exists(REFLAYOUT), "Fortran driver not built (run __graft_entry__.build())"
+    o = O.SG4(4, 4, LB=3)
+    rng = np.random.default_rng(41)
+    n, nb = o.sum_nq, o.nb_ba
+    V11, V22, V12 = rng.standard_normal(n), rng.standard_normal(n), 0.3 * rng.standard_normal(n)
+    GridV = np.zeros((n, 2, 2), order="F")
+    GridV[:, 0, 0], GridV[:, 1, 1], GridV[:, 0, 1], GridV[:, 1, 0] = V11, V22, V12, V12
+    ck = -rng.uniform(0.2, 1.0, o.D)
+    psi = rng.standard_normal((2 * nb, 5))
+    fin, fout = tmp_path / "tables.bin", tmp_path / "hpsi.bin"
+    _write_reference_layout(fin, o, 21, [GridV, ck], psi, max_nvec=2)
+    r = subprocess.run([REFLAYOUT, str(fin), str(fout)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = fout.read_bytes()
+    m = 2 * nb * 5
+    out = np.frombuffer(raw[:16 * m], dtype=np.float64)
+    host, viapsi = out[:m].reshape(5, 2 * nb).T, out[m:].reshape(5, 2 * nb).T
+    status = int(np.frombuffer(raw[16 * m:], dtype=np.int32)[0])
+    z = np.zeros(o.D)
+    ref = np.vstack([o.hpsi(psi[:nb], V11, ck) + o.hpsi(psi[nb:], V12, z),
+                     o.hpsi(psi[nb:], V22, ck) + o.hpsi(psi[:nb], V12, z)])
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(host - ref)) / scale < 1e-12
+    assert np.max(np.abs(viapsi - ref)) / scale < 1e-12
+    assert status == -4                                        # SG4_ERR_UNSUPPORTED
