@@ -1008,7 +1008,7 @@ static int build_mfma(sg4_plan* P) {
   return SG4_OK;
 }
 
-// Work items of the MFMA path: per term, chunks of nv = 2^g <= 16 vectors
+// Work items of the MFMA path: per term, chunks of nv = 2^g <= 64 vectors
 // with (4 RS + 8) N + 256 bytes of LDS (the FP32 tensor with row stride RS,
 // the FP32 potential, the map, scratch words) within the budget; a term that needs more than the budget with one
 // vector goes to the second list, launched with the full LDS.
@@ -1033,7 +1033,7 @@ static int get_items_mfma(sg4_plan* P, int nvec, int budget, int xcd, ItemPlan**
     const bool large = need(0) > budget;
     const int cap = large ? LDS_MAX : budget;
     int gmax = 0;
-    while ((1 << (gmax + 1)) <= std::min(nvec, 16) && need(gmax + 1) <= cap) ++gmax;
+    while ((1 << (gmax + 1)) <= std::min(nvec, 64) && need(gmax + 1) <= cap) ++gmax;
     double c1 = 0, fm1 = 0;   // per vector: cost model, MFMA flops
     for (int g = 0; g < M.G; ++g) {
       const double kp = M.K[g] <= 16 ? 16.0 : 32.0;
@@ -1168,8 +1168,9 @@ static int prepare(sg4_plan* P, int nvec, int prec, const LaunchCfg& cfg, ItemPl
 }
 
 // Width of the rows-layout staging block of a column-major apply of nvec
-// vectors.  Work items carry 2^g <= 16 vectors and a term's item costs little
-// more with 16 vectors than with 8 (its passes are latency-bound), so the
+// vectors.  Work items carry 2^g vectors (as many as the LDS budget allows)
+// and a term's item costs little more with 16 vectors than with 8 (its
+// passes are latency-bound), so the
 // block is padded with zero vectors to the next power of two (nvec <= 16) or
 // multiple of 16 when the plan's max_nvec allows: 9 vectors, say, then run as
 // one 16-vector item per term instead of an 8- and a 1-vector item (10D L=8:

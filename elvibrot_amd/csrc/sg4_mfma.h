@@ -161,10 +161,10 @@ __device__ __forceinline__ int slot_k(int j, int s) {
 enum { MIN_LDS = 0, MIN_GATHER = 1 };
 enum { MOUT_LDS = 0, MOUT_SCATTER = 1 };
 
-// LDS row stride (floats) of an item with nv vectors: 20 for 16 vectors, so
-// that the four lane groups of a B-operand read (rows 4A apart, A odd for
-// odd mode sizes) fall in distinct banks; nv otherwise.
-__host__ __device__ constexpr int mrow_stride(int nv) { return nv == 16 ? 20 : nv; }
+// LDS row stride (floats) of an item with nv vectors: nv + 4 from 16 vectors
+// on (20, 36, 68), so that the four lane groups of a B-operand read (rows 4A
+// apart, A odd for odd mode sizes) fall in distinct banks; nv below 16.
+__host__ __device__ constexpr int mrow_stride(int nv) { return nv >= 16 ? nv + 4 : nv; }
 
 struct MCtx {
   float* X;                 // LDS tensor, element (p, v) at X[p*RS + v]

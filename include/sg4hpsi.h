@@ -190,8 +190,8 @@ int sg4_plan_set_jacobi_triatomic(sg4_plan *p, const double *prm);
  * before anything is enqueued.
  * The block is staged in the rows layout padded with zero vectors to the next
  * power of two (nvec <= 16) or multiple of 16 within max_nvec: the kernels
- * carry 2^g <= 16 vectors per work item, and one wider item per term costs
- * less than several narrow ones. */
+ * carry 2^g vectors per work item (as many as the LDS budget allows), and
+ * one wider item per term costs less than several narrow ones. */
 int sg4_apply(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, int32_t prec);
 /* Allocation-free variant of sg4_apply, for callers that cannot allocate
  * device memory (code under HIP graph capture, serving loops, harnesses that

@@ -74,6 +74,22 @@ def test_f32x2_general_potential_absent_rows(cuda):
     assert e2 < TOL2 and rc < KCOND
 
 
+@pytest.mark.parametrize("D,L,nvec", [(6, 6, 32), (6, 6, 24), (4, 5, 32), (10, 4, 32), (4, 5, 64), (6, 4, 48)])
+def test_f32x2_wide_blocks(cuda, D, L, nvec):
+    """Blocks of 24 to 64 vectors (work items of up to 64 vectors where the
+    LDS budget allows) at the stated tolerance of the mode."""
+    b = E.SG4Basis(D, L)
+    op = E.SG4Operator(b, max_nvec=64, potential=("henon_heiles", 0.111803))
+    o = O.SG4(D, L)
+    V = o.grid_hh(0.111803)
+    psi = np.random.default_rng(7 * D + L + nvec).standard_normal((b.nb_ba, nvec))
+    got, ref = op.apply_host(psi, E.PREC_F32X2), o.hpsi(psi, V, -0.5 * np.ones(D))
+    e2, em = errs(got, ref)
+    rc = cond_ratio(got, ref, o, psi, V, D)
+    print(f"D={D} L={L} nvec={nvec}: 2-norm {e2:.2e} max {em:.2e} ratio {rc:.3f}")
+    assert e2 < TOL2 and rc < KCOND
+
+
 def test_f32x2_batch_of_uneven_instances(cuda):
     """One batched plan over instances of different D and L (the layout of the
     64-instance propagation batch): each instance's rows of the split-precision
