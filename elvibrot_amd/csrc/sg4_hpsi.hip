@@ -328,8 +328,11 @@ struct sg4_plan {
   int nder = 0;
   sg4::DevBuf d_der, d_F;
   // split type-1 path: kinetic sparse matrix K (CSR over packed rows) and the
-  // host copies of the map and matrices it is assembled from
+  // host copies of the map and matrices it is assembled from; split_ok: K is
+  // built (the algebra holds), split_fp: the FP64/FP32 term kernels also
+  // have a register-blocked shape for every term and fit its LDS
   bool split_ok = false;
+  bool split_fp = false;
   int64_t kin_nnz = 0;
   std::vector<int32_t> h_map;
   std::vector<double> h_mats;
@@ -445,14 +448,17 @@ static double binom(int n, int k) {
 // integrals that vanish exactly (odd parity, and |i - j| > 2 for HO bases;
 // they are ~1e-16 relative, the others >= 1e-2) and are not stored.
 // The split path is enabled only when every 1-D basis has B^T W B = I to
-// 1e-11, the terms are exactly the Smolyak set {L-D+1 <= |l| <= L} with
-// these weights, and every term is square and grouped.
+// 1e-11 and the terms are exactly the Smolyak set {L-D+1 <= |l| <= L} with
+// these weights; the FP64/FP32 split kernels need in addition a shape for
+// every term's mode groups (split_fp), the matrix-core path does not.
 static int build_kinetic(sg4_plan* P) {
   P->split_ok = false;
+  P->split_fp = false;
   if (P->type_op != 1 || !P->grouping) return SG4_OK;
+  bool fp = true;
   for (const TermDev& T : P->h_terms) {
-    if (T.nact > 0 && T.ngrp == 0) return SG4_OK;
-    if (20.0 * T.nq > (double)LDS_MAX) return SG4_OK;   // FP64: one vector + potential + map
+    if (T.nact > 0 && T.ngrp == 0) fp = false;
+    if (20.0 * T.nq > (double)LDS_MAX) fp = false;   // FP64: one vector + potential + map
   }
   const size_t nmd = P->h_md.size();
   std::vector<int64_t> toff(nmd);
@@ -602,6 +608,7 @@ static int build_kinetic(sg4_plan* P) {
   HIP_TRY(hipMemcpy(P->d_kval.p, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice));
   P->kin_nnz = (int64_t)col.size();
   P->split_ok = true;
+  P->split_fp = fp;
   return SG4_OK;
 }
 
@@ -853,6 +860,7 @@ static int grid_blocks(int64_t n) {
 }
 
 static bool use_split(const sg4_plan* P);
+static bool use_split_alg(const sg4_plan* P);
 
 // bf16 limbs of the group matrices (round to nearest even)
 static uint16_t to_bf16(double d) {
@@ -880,7 +888,7 @@ static double from_bf16(uint16_t h) {
 static int build_mfma(sg4_plan* P) {
   if (P->mfma_valid) return SG4_OK;
   if (P->no_alloc) return not_prepared();
-  if (!use_split(P)) {
+  if (!use_split_alg(P)) {
     set_error("the split-precision MFMA mode needs the split type-1 path (square HO/Legendre "
               "modes with exact quadrature, complete Smolyak term set)");
     return SG4_ERR_UNSUPPORTED;
@@ -1117,8 +1125,17 @@ static int check_apply(sg4_plan* P, int nvec, int prec) {
   return SG4_OK;
 }
 
-static bool use_split(const sg4_plan* P) {
+// The split algebra (kinetic part as K, terms keep the potential part): the
+// matrix-core path needs only this; the FP64/FP32 term kernels also need
+// their shapes (use_split).
+static bool use_split_alg(const sg4_plan* P) {
   return P->type_op == 1 && P->grouping && P->split_ok;
+}
+
+static bool use_split(const sg4_plan* P) { return use_split_alg(P) && P->split_fp; }
+
+static bool use_split_for(const sg4_plan* P, int prec) {
+  return prec == SG4_PREC_F32X2 ? use_split_alg(P) : use_split(P);
 }
 
 // Launch configuration for (nvec, prec): the one tuned or set for it, else
@@ -1145,7 +1162,7 @@ static LaunchCfg cfg_for(const sg4_plan* P, int nvec, int prec) {
 static int prepare(sg4_plan* P, int nvec, int prec, const LaunchCfg& cfg, ItemPlan** ip) {
   int rc;
   if (P->deterministic) {
-    if (!use_split(P)) {
+    if (!use_split_for(P, prec)) {
       set_error("deterministic accumulation is available on the split type-1 path only");
       return SG4_ERR_UNSUPPORTED;
     }
@@ -1194,7 +1211,7 @@ static int apply_rows(sg4_plan* P, int nvec, const double* psi, double* hpsi, in
   if ((rc = prepare(P, nvec, prec, cfg, &ip))) return rc;
   const int esize = (prec == SG4_PREC_FP64) ? 8 : 4;
   const int64_t n = P->nb_ba * nvec;
-  const bool split = use_split(P);
+  const bool split = use_split_for(P, prec);
   if (!split) hipLaunchKernelGGL(k_fill, dim3(grid_blocks(n)), dim3(256), 0, P->stream, hpsi, n, 0.0);
   KArgs a;
   a.terms = P->d_terms.as<TermDev>();

@@ -40,7 +40,8 @@ extern "C" {
 #define SG4_PREC_FP64 0      /* FP64 throughout (parity mode, 1e-12 relative) */
 #define SG4_PREC_F32 1       /* mode contractions in FP32, FP64 gather/scatter */
 #define SG4_PREC_F32X2 2     /* split-precision contractions on the matrix cores (split
-                                 type-1 path only): FP32 operands as three bf16 limbs
+                                 type-1 algebra only, modes of <= 32 points):
+                                 FP32 operands as three bf16 limbs
                                  (hi + mid + lo), six limb products per product on
                                  v_mfma_f32_16x16x{16,32}_bf16 with FP32 accumulation:
                                  FP32-accurate contractions; FP64 kinetic part, term sum
@@ -379,8 +380,11 @@ int sg4_apply_graph(sg4_plan *p, int32_t nvec, const double *psi, double *hpsi, 
  * atomics.  Costs 16 bytes of device memory per result element. */
 #define SG4_OPT_DETERMINISTIC 2
 int sg4_plan_set_option(sg4_plan *p, int32_t option, int32_t value);
-/* Whether applies of this plan take the split path (split = 1) and the number
- * of stored entries of its kinetic matrix.  No counterpart in the reference. */
+/* Whether FP64 and FP32 applies of this plan take the split path (split = 1)
+ * and the number of stored entries of its kinetic matrix (nonzero whenever
+ * the split algebra holds, which is all SG4_PREC_F32X2 needs; the FP64/FP32
+ * split kernels also need a register-blocked shape for every term, e.g. no
+ * mode of more than 24 points).  No counterpart in the reference. */
 int sg4_plan_split_info(const sg4_plan *p, int32_t *split, int64_t *kin_nnz);
 
 /* Statistics of the last apply for roofline reporting: its work items, and

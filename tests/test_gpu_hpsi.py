@@ -36,6 +36,35 @@ def test_fp64_parity_host_entry(cuda, D, L, nvec):
     assert rel(got, ref) < TOL
 
 
+@pytest.mark.parametrize("D,L", [(1, 0), (1, 3), (1, 6), (1, 15), (3, 0), (5, 0), (2, 1), (2, 12)])
+@pytest.mark.parametrize("nvec", [3, 17])
+def test_smallest_sets_all_precisions(cuda, D, L, nvec):
+    """The edge of the term sets: one mode (the harmonic oscillator,
+    diag(n + 1/2); 31 points at L = 15, the widest single-mode tile of the
+    matrix cores), L = 0 (one function on the one-point grid: D/2), 2D L = 1,
+    and 2D L = 12 (modes of 25 points).  Modes of more than 24 points take
+    the grid path in FP64 and FP32 and the split algebra on the matrix cores.
+    FP64 to the parity bar and the closed form; the reduced modes within 2e-6
+    (2-norm) of the oracle."""
+    print(f"case D={D} L={L} nvec={nvec}", flush=True)
+    b, op, o, V = hh_pair(D, L, max_nvec=32)
+    split, nnz = op.split_info()
+    assert nnz > 0 and split == (2 * L + 1 <= 24)
+    rng = np.random.default_rng(7 + 10 * D + L)
+    psi = rng.standard_normal((b.nb_ba, nvec))
+    ref = o.hpsi(psi, V, -0.5 * np.ones(D))
+    got = op.apply_host(psi)
+    assert rel(got, ref) < TOL
+    if o.nterms == 1:
+        diag = np.full(b.nb_ba, D / 2) if L == 0 else np.arange(b.nb_ba) + 0.5
+        assert rel(got, diag[:, None] * psi) < TOL
+    for prec in (E.PREC_F32, E.PREC_F32X2):
+        g = op.apply_host(psi, prec)
+        e2 = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+        print(f"prec {prec}: 2-norm {e2:.2e}")
+        assert e2 < 2e-6
+
+
 def test_fp64_parity_device_entries(cuda):
     import torch
     D, L, nvec = 6, 6, 16

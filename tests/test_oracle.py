@@ -137,6 +137,23 @@ def test_smolyak_exact_for_separable_polynomial_potential():
     assert abs(h[0] - np.sum(a / 2 + 0.25)) < 1e-12
 
 
+@pytest.mark.parametrize("D,L", [(1, 0), (1, 3), (1, 6), (1, 15), (3, 0), (5, 0)])
+def test_smallest_sets_closed_forms(D, L):
+    """One mode: the only term is l = (L) with weight 1 and the Henon-Heiles
+    potential is q^2/2, so H is the harmonic oscillator in its own basis,
+    diag(n + 1/2), exact because the 2L+1 Gauss-Hermite points integrate
+    every product the matrix needs.  L = 0: one function (0,...,0) on the
+    one-point grid q = 0, where V = 0 and -phi0''/2 = phi0/2 per mode: D/2."""
+    s = O.SG4(D, L)
+    assert s.nterms == 1
+    H = s.hpsi(np.eye(s.nb_ba), s.grid_hh(0.111803), -0.5 * np.ones(D))
+    if L == 0:
+        assert H.shape == (1, 1) and abs(H[0, 0] - D / 2) < 1e-14
+    else:
+        assert s.nb_ba == 2 * L + 1
+        assert np.max(np.abs(H - np.diag(np.arange(s.nb_ba) + 0.5))) < 1e-12
+
+
 def _dense_H(s, V, ck):
     H = s.hpsi(np.eye(s.nb_ba), V, ck)
     return 0.5 * (H + H.T), np.max(np.abs(H - H.T))
