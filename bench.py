@@ -49,6 +49,9 @@ UNIT = {"fp64": "FP64 VALU (= FP64 MFMA peak)", "f32": "FP32 VALU",
 # equal levels in modes 3..D-1 on one XCD; 2: modes 2..D-1)
 PINNED = {(10, 8, 16): {"fp64": (512, 163840, 2048, 3), "f32": (768, 163840, 2048, 2),
                         "f32x2": (1024, 163840, 512, 2)}}
+# the propagation workload's 2-column applies (64-instance batch below):
+# the autotuner's best split-precision launch and the best FP64 one measured
+PINNED_PROP = {"f32x2": (256, 32768, 2048, 3), "fp64": (256, 32768, 0, 2)}
 PREC = {"fp64": 0, "f32": 1, "f32x2": 2}
 DTYPE = {"fp64": "f64", "f32": "f32", "f32x2": "bf16x3-split/f32-acc"}
 KERNEL = {"fp64": "k_hpsi_t1s_f64", "f32": "k_hpsi_t1s_f32", "f32x2": "k_hpsi_mfma"}
@@ -123,10 +126,14 @@ def propagation_bench(E, torch, args):
     eps = np.random.default_rng(7).normal(0.0, 0.01, 200)
     mu = op.grid_dipole()
     setup = time.perf_counter() - t0
-    for nv in (2, 4):
-        op.autotune(nv, prec)
-        if prec != PREC["fp64"]:
-            op.autotune(nv, PREC["fp64"])
+    if args.autotune or args.prec not in PINNED_PROP:
+        for nv in (2, 4):
+            op.autotune(nv, prec)
+            if prec != PREC["fp64"]:
+                op.autotune(nv, PREC["fp64"])
+    else:       # pinned 2-column launches; the 4-column applies inherit them
+        for pn in {args.prec, "fp64"}:
+            op.set_config(2, PREC[pn], *PINNED_PROP[pn])
     lo, hi = spectral_bounds(op)
     w = 0.05 * (hi - lo) + np.abs(eps).max() * mu.abs().max().item()
     prop = ChebyshevPropagator(op, 0.1, lo - w, hi + w, mu=mu, prec=prec)
