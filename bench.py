@@ -50,8 +50,8 @@ UNIT = {"fp64": "FP64 VALU (= FP64 MFMA peak)", "f32": "FP32 VALU",
 PINNED = {(10, 8, 16): {"fp64": (512, 163840, 2048, 3), "f32": (768, 163840, 2048, 2),
                         "f32x2": (1024, 163840, 512, 2)}}
 # the propagation workload's 2-column applies (64-instance batch below):
-# the autotuner's best split-precision launch and the best FP64 one measured
-PINNED_PROP = {"f32x2": (256, 32768, 2048, 3), "fp64": (256, 32768, 0, 2)}
+# sg4_plan_autotune's best launches, timed over whole applies on MI355X
+PINNED_PROP = {"f32x2": (256, 32768, 2048, 3), "fp64": (256, 65536, 0, 3)}
 PREC = {"fp64": 0, "f32": 1, "f32x2": 2}
 DTYPE = {"fp64": "f64", "f32": "f32", "f32x2": "bf16x3-split/f32-acc"}
 KERNEL = {"fp64": "k_hpsi_t1s_f64", "f32": "k_hpsi_t1s_f32", "f32x2": "k_hpsi_mfma"}
