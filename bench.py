@@ -108,6 +108,17 @@ def measured_traffic(kernel, D, L, nvec):
     return float(m.group(1)) if m else None
 
 
+def reference_flops(b, nvec):
+    """Flops of one block apply by the reference's algorithm (B->G, kinetic
+    on the grid, G->B; sub_OpPsi_SG4.f90:1352-1666): F1 = 3 sum_t S_t per
+    vector, S_t = sum_k 2 N_t n_k over the terms t of the basis b."""
+    D, L = b.D, b.L
+    lev = b.levels()
+    ntab = {(k, l): b.basis1d(k, l)["nq"] for k in range(D) for l in range(L + 1)}
+    nq = np.array([[ntab[k, int(l)] for k, l in enumerate(row)] for row in lev], dtype=np.float64)
+    return 3.0 * float(np.sum(2.0 * np.prod(nq, axis=1) * nq.sum(axis=1))) * nvec
+
+
 def propagation_bench(E, torch, args):
     """64 uneven instances (D ~ U{4..12}, L ~ U{4..8}, numpy seed 7) in one
     batched plan, dipole sum_i q_i, 200 piecewise-constant N(0, 0.01)
@@ -303,12 +314,7 @@ def main():
             torch.cuda.synchronize()
             tg = time.perf_counter() - t0
             graph[pname] = {"H·psi/s": nvec * args.steps / tg, "ms_per_step": tg / args.steps * 1e3}
-    # flops of the reference's algorithm (B->G, kinetic on the grid, G->B):
-    # F1 = 3 sum_t S_t per vector, S_t = sum_k 2 N_t n_k
-    lev = b.levels()
-    ntab = {(k, l): b.basis1d(k, l)["nq"] for k in range(D) for l in range(L + 1)}
-    nq = np.array([[ntab[k, int(l)] for k, l in enumerate(row)] for row in lev], dtype=np.float64)
-    F1 = 3.0 * float(np.sum(2.0 * np.prod(nq, axis=1) * nq.sum(axis=1))) * nvec
+    F1 = reference_flops(b, nvec)
     kname = KERNEL[args.prec] if split or args.prec == "f32x2" else (
         "k_hpsi_t1_" + ("f64" if args.prec == "fp64" else "f32"))
     traffic = measured_traffic(kname, D, L, nvec)
