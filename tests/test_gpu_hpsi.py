@@ -335,6 +335,39 @@ def test_graph_replays_match_eager(cuda):
         assert torch.max(torch.abs(Z - (r + 1) * ref)).item() <= 1e-12 * torch.max(torch.abs(Z)).item()
 
 
+@pytest.mark.parametrize("D,L,nvec,prec", [(6, 4, 5, E.PREC_FP64), (6, 4, 5, E.PREC_F32X2),
+                                           (10, 8, 16, E.PREC_FP64), (10, 8, 16, E.PREC_F32X2)])
+def test_roofline_numerators_from_the_term_set(cuda, D, L, nvec, prec):
+    """The algorithmic flops and bytes per launch that the bench line divides
+    by the kernel time (sg4_plan_stats after an apply), recounted from the
+    basis tables: on the split path every term costs per vector 4 N_t n_k
+    flops per active mode (n_k > 1: a basis->grid and a grid->basis product)
+    plus N_t for the potential; the compulsory bytes are the FP64 potential
+    (8 per grid point), the int32 map (4 per term row) and one read of psi
+    plus one write of H.psi.  At 10D L=8 x16: 42.29 GFLOP and 0.699 GB."""
+    import torch
+    b = E.SG4Basis(D, L)
+    op = E.SG4Operator(b, max_nvec=nvec)
+    if (D, L) == (10, 8):      # the launches the bench pins: no item beyond the LDS budget
+        op.set_config(nvec, E.PREC_FP64, 512, 163840, 2048, 3)
+        op.set_config(nvec, E.PREC_F32X2, 1024, 163840, 512, 2)
+    x = torch.randn(nvec, b.nb_ba, device=cuda, dtype=torch.float64)
+    y = torch.empty_like(x)
+    op.apply(x, y, prec)
+    torch.cuda.synchronize()
+    st = op.stats(nvec)
+    nq = {(k, l): b.basis1d(k, l)["nq"] for k in range(D) for l in range(L + 1)}
+    n = np.array([[nq[k, int(l)] for k, l in enumerate(row)] for row in b.levels()], dtype=np.float64)
+    N = np.prod(n, axis=1)
+    flops = nvec * np.sum(4.0 * N * np.where(n > 1, n, 0.0).sum(axis=1) + N)
+    nbytes = 8.0 * N.sum() + 4.0 * N.sum() + 16.0 * b.nb_ba * nvec
+    print(f"D={D} L={L} x{nvec} prec {prec}: {st['flops']:.6g} flops, {st['bytes']:.6g} bytes")
+    assert st["flops"] == pytest.approx(flops, rel=1e-12)
+    assert st["bytes"] == pytest.approx(nbytes, rel=1e-12)
+    if (D, L, nvec) == (10, 8, 16):
+        assert flops == pytest.approx(42.29e9, rel=1e-3) and nbytes == pytest.approx(0.699e9, rel=1e-3)
+
+
 def test_graph_replay_10d_L8_block16(cuda):
     """The headline block through one captured HIP graph (kinetic rows, term
     kernel and, with the deterministic sum, its clear and finish kernels):
