@@ -4,17 +4,21 @@
 //
 //   gram:    C (m x q) = A B^T, A: m rows, B: q rows (stride lda, ldb), the
 //            inner dimension n long.  Each workgroup takes 32-column tiles in
-//            a grid-stride loop, stages the rows of a tile in LDS and keeps
-//            4 x 4 register blocks of C (up to two per thread); the
+//            a grid-stride loop, stages the rows of a tile in LDS (sized for
+//            this m and q, so that small grams keep several workgroups per
+//            CU and their tile loads in flight) and keeps 4 x 4 register
+//            blocks of C: with fewer blocks than threads the tile's columns
+//            are dealt to slices of threads, summed in LDS at the end; the
 //            workgroup partials go to a work array and a second kernel adds
 //            them, one wave per output, in a fixed order (deterministic, no
 //            atomics).
 //   combine: Y (k x n) = beta Y + U X, U (k x d) small and staged in LDS,
 //            X: d rows; one column per thread, coalesced across the wave,
 //            U broadcast from LDS.
-// Measured at 10D L=8 (n = 1.26 M): gram 8 x 8 0.22 ms, 20 x 20 0.28 ms,
-// 16 x 84 0.67 ms (HBM floor 0.02, 0.05, 0.13 ms: each tile waits for its
-// loads behind a barrier at two workgroups per CU); combine 8 x 8 0.05 ms,
+// Measured at 10D L=8 (n = 1.26 M): gram 8 x 8 0.065 ms, 16 x 20 0.16 ms,
+// 20 x 20 0.19 ms, 4 x 84 0.32 ms (HBM floor 0.02, 0.05, 0.05, 0.11 ms; with
+// LDS for 124 rows at two workgroups per CU and no column slices: 0.22, 0.28 ms
+// for 8 x 8 and 20 x 20); combine 8 x 8 0.05 ms,
 // 16 x 48 0.20 ms, 20 x 84 0.48 ms.  The Davidson driver uses them for the
 // small grams and rotations and the library GEMM for the others.
 #include <hip/hip_runtime.h>
@@ -28,21 +32,36 @@ namespace sg4 {
 namespace {
 
 constexpr int GT = 32;         // columns per gram tile
-constexpr int GMAX = 124;      // rows of A or B (LDS rows padded to GMAX + 1: 64000 bytes)
+constexpr int GMAX = 124;      // rows of A or B (LDS: 32 x (4 ceil(rows/4) + 1) doubles each)
 constexpr int GBLK = 512;      // 4 x 4 output blocks per workgroup (two per thread)
-constexpr int GNB = 512;       // workgroups of the gram's first kernel (at most)
+constexpr int GNB = 2048;      // workgroups of the gram's first kernel (at most)
 constexpr int CMAXU = 8192;    // entries of U in LDS (64 KiB)
 constexpr int CK = 16;         // outputs of the combine per pass over X
+
+// column slices of the gram's workgroup for nblk 4 x 4 output blocks
+__host__ __device__ constexpr int gram_slices(int nblk) {
+  return nblk >= 256 ? 1 : (256 / nblk < GT ? 256 / nblk : GT);
+}
 
 __global__ __launch_bounds__(256) void k_gram_part(int m, int q, int64_t n, const double* __restrict__ A,
                                                    int64_t lda, const double* __restrict__ B, int64_t ldb,
                                                    double* __restrict__ part) {
-  // As[c][i]: column c of the tile, row i (rows padded to a multiple of 4 with zeros)
-  __shared__ double As[GT][GMAX + 1];
-  __shared__ double Bs[GT][GMAX + 1];
+  // As[c pa + i]: column c of the tile, row i (rows padded to a multiple of 4
+  // with zeros; pitch pa = 4 m4 + 1), Bs likewise with pitch pb = 4 q4 + 1
+  extern __shared__ double gl[];
   const int m4 = (m + 3) >> 2, q4 = (q + 3) >> 2, nblk = m4 * q4;
+  const int pa = 4 * m4 + 1, pb = 4 * q4 + 1;
+  double* As = gl;
+  double* Bs = gl + GT * pa;
   const int tid = threadIdx.x;
-  const int b0 = tid, b1 = tid + 256;          // this thread's output blocks
+  // with fewer than 256 output blocks the columns of a tile are dealt to ns
+  // slices of threads (block tid % nblk, slice tid / nblk: columns slice,
+  // slice + ns, ...), whose sums are added in LDS at the end in slice order;
+  // otherwise a thread keeps output blocks tid and tid + 256 over all columns
+  const int ns = gram_slices(nblk);
+  const int b0 = ns > 1 ? tid % nblk : tid, b1 = tid + 256;
+  const int sl = ns > 1 ? tid / nblk : 0;
+  const bool on0 = ns > 1 ? tid < ns * nblk : b0 < nblk;
   double acc0[16], acc1[16];
 #pragma unroll
   for (int e = 0; e < 16; ++e) { acc0[e] = 0.0; acc1[e] = 0.0; }
@@ -52,31 +71,31 @@ __global__ __launch_bounds__(256) void k_gram_part(int m, int q, int64_t n, cons
     __syncthreads();                            // previous tile consumed
     for (int e = tid; e < GT * 4 * m4; e += 256) {
       const int i = e / GT, c = e % GT;         // consecutive threads: consecutive columns
-      As[c][i] = (i < m && c0 + c < n) ? A[(int64_t)i * lda + c0 + c] : 0.0;
+      As[c * pa + i] = (i < m && c0 + c < n) ? A[(int64_t)i * lda + c0 + c] : 0.0;
     }
     for (int e = tid; e < GT * 4 * q4; e += 256) {
       const int i = e / GT, c = e % GT;
-      Bs[c][i] = (i < q && c0 + c < n) ? B[(int64_t)i * ldb + c0 + c] : 0.0;
+      Bs[c * pb + i] = (i < q && c0 + c < n) ? B[(int64_t)i * ldb + c0 + c] : 0.0;
     }
     __syncthreads();
-    if (b0 < nblk) {
+    if (on0) {
       const int bi = (b0 / q4) * 4, bj = (b0 % q4) * 4;
-      for (int c = 0; c < GT; ++c) {
+      for (int c = sl; c < GT; c += ns) {
         double a[4], b[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) { a[u] = As[c][bi + u]; b[u] = Bs[c][bj + u]; }
+        for (int u = 0; u < 4; ++u) { a[u] = As[c * pa + bi + u]; b[u] = Bs[c * pb + bj + u]; }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
           for (int v = 0; v < 4; ++v) acc0[4 * u + v] = fma(a[u], b[v], acc0[4 * u + v]);
       }
     }
-    if (b1 < nblk) {
+    if (ns == 1 && b1 < nblk) {
       const int bi = (b1 / q4) * 4, bj = (b1 % q4) * 4;
       for (int c = 0; c < GT; ++c) {
         double a[4], b[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) { a[u] = As[c][bi + u]; b[u] = Bs[c][bj + u]; }
+        for (int u = 0; u < 4; ++u) { a[u] = As[c * pa + bi + u]; b[u] = Bs[c * pb + bj + u]; }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -86,6 +105,22 @@ __global__ __launch_bounds__(256) void k_gram_part(int m, int q, int64_t n, cons
   }
   // partial of output e = i q + j at part[e * nb + workgroup]
   const int64_t nb = gridDim.x;
+  if (ns > 1) {
+    __syncthreads();                            // the last tile consumed: LDS reused
+    double* red = gl;                           // [slice][block][16]
+    if (on0)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) red[(sl * nblk + b0) * 16 + e] = acc0[e];
+    __syncthreads();
+    for (int o = tid; o < nblk * 16; o += 256) {
+      const int b = o >> 4, e = o & 15;
+      double sum = 0.0;
+      for (int k = 0; k < ns; ++k) sum += red[(k * nblk + b) * 16 + e];
+      const int i = (b / q4) * 4 + (e >> 2), j = (b % q4) * 4 + (e & 3);
+      if (i < m && j < q) part[((int64_t)i * q + j) * nb + blockIdx.x] = sum;
+    }
+    return;
+  }
   if (b0 < nblk) {
     const int bi = (b0 / q4) * 4, bj = (b0 % q4) * 4;
     for (int u = 0; u < 4; ++u)
@@ -192,7 +227,12 @@ int sg4_block_gram(sg4_plan* p, int32_t m, int32_t q, int64_t n, const double* A
   const hipStream_t st = (hipStream_t)plan_stream(p);
   const int64_t ntile = (n + GT - 1) / GT;
   const int nb = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)GNB, ntile, nwork / ((int64_t)m * q)}));
-  hipLaunchKernelGGL(k_gram_part, dim3(nb), dim3(256), 0, st, m, q, n, A, lda, B, ldb, work);
+  // LDS: the tile's rows, or the slice sums at the end if larger (<= 32 KiB)
+  const int m4 = (m + 3) / 4, q4 = (q + 3) / 4, ns = gram_slices(m4 * q4);
+  const size_t tile = sizeof(double) * GT * ((size_t)(4 * m4 + 1) + (4 * q4 + 1));
+  const size_t red = ns > 1 ? sizeof(double) * 16 * (size_t)ns * m4 * q4 : 0;
+  const size_t lds = std::max(tile, red);
+  hipLaunchKernelGGL(k_gram_part, dim3(nb), dim3(256), lds, st, m, q, n, A, lda, B, ldb, work);
   BK_TRY(hipGetLastError());
   const int mq = m * q;
   hipLaunchKernelGGL(k_gram_sum, dim3((mq + 3) / 4), dim3(256), 0, st, mq, nb, work, C);

@@ -311,7 +311,7 @@ int sg4_taylor_propagate(sg4_plan *H, const double *mu, int32_t ns, double *psi,
  * sg4_block_gram: C (m x q, row-major) = A B^T, m, q <= 124 with
  *   ceil(m/4) ceil(q/4) <= 512; work: nwork doubles of scratch (at least
  *   m q; sg4_block_gram_work gives the size that lets every workgroup keep
- *   its partial, 512 m q); partial sums are added in a fixed order, so the
+ *   its partial, 2048 m q); partial sums are added in a fixed order, so the
  *   result is reproducible bit for bit.
  * sg4_block_combine: Y (k x n) = beta Y + U X, U (k x d, row-major, DEVICE),
  *   X d rows, k d <= 8192; Y must not overlap X; with beta = 0, Y is not
