@@ -12,15 +12,16 @@
 //            workgroup partials go to a work array and a second kernel adds
 //            them, one wave per output, in a fixed order (deterministic, no
 //            atomics).
-//   combine: Y (k x n) = beta Y + U X, U (k x d) small and staged in LDS,
-//            X: d rows; one column per thread, coalesced across the wave,
-//            U broadcast from LDS.
+//   combine: Y (k x n) = beta Y + U X, U (k x d) small, X: d rows; one
+//            column per thread, coalesced across the wave, U entries from
+//            scalar loads (wave-uniform addresses) as FMA operands.
 // Measured at 10D L=8 (n = 1.26 M): gram 8 x 8 0.065 ms, 16 x 20 0.16 ms,
 // 20 x 20 0.19 ms, 4 x 84 0.32 ms (HBM floor 0.02, 0.05, 0.05, 0.11 ms; with
 // LDS for 124 rows at two workgroups per CU and no column slices: 0.22, 0.28 ms
-// for 8 x 8 and 20 x 20); combine 8 x 8 0.05 ms,
-// 16 x 48 0.20 ms, 20 x 84 0.48 ms.  The Davidson driver uses them for the
-// small grams and rotations and the library GEMM for the others.
+// for 8 x 8 and 20 x 20); combine 8 x 8 0.031 ms, 16 x 48 0.16 ms, 20 x 84
+// 0.34 ms, 36 x 84 0.69 ms (U broadcast from LDS instead: 0.05, 0.20, 0.48,
+// 0.83 ms).  The Davidson driver uses them for the small grams and the
+// rotations to at most 20 rows, and the library GEMM for the others.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,8 +36,9 @@ constexpr int GT = 32;         // columns per gram tile
 constexpr int GMAX = 124;      // rows of A or B (LDS: 32 x (4 ceil(rows/4) + 1) doubles each)
 constexpr int GBLK = 512;      // 4 x 4 output blocks per workgroup (two per thread)
 constexpr int GNB = 2048;      // workgroups of the gram's first kernel (at most)
-constexpr int CMAXU = 8192;    // entries of U in LDS (64 KiB)
-constexpr int CK = 16;         // outputs of the combine per pass over X
+constexpr int CMAXU = 8192;    // entries of U (k d) at most
+constexpr int CK = 16;         // outputs of the combine per pass over X (k <= 16)
+constexpr int CKW = 48;        // the same for wider k (k > 48: several passes)
 
 // column slices of the gram's workgroup for nblk 4 x 4 output blocks
 __host__ __device__ constexpr int gram_slices(int nblk) {
@@ -149,12 +151,13 @@ __global__ __launch_bounds__(256) void k_gram_sum(int mq, int nb, const double* 
   if (lane == 0) C[e] = s;
 }
 
+// Y rows i0..i0+CK-1 for the columns of this thread; U entries are read
+// straight from global memory at wave-uniform addresses (scalar loads, one
+// SGPR operand per FMA), X once per pass.
+template <int CK>
 __global__ __launch_bounds__(256) void k_combine(int k, int d, int64_t n, const double* __restrict__ U,
                                                  const double* __restrict__ X, int64_t ldx, double beta,
                                                  double* __restrict__ Y, int64_t ldy) {
-  extern __shared__ double Us[];
-  for (int e = threadIdx.x; e < k * d; e += blockDim.x) Us[e] = U[e];
-  __syncthreads();
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     for (int i0 = 0; i0 < k; i0 += CK) {
       double acc[CK];
@@ -166,16 +169,18 @@ __global__ __launch_bounds__(256) void k_combine(int k, int d, int64_t n, const 
 #pragma unroll
         for (int t = 0; t < 4; ++t) x[t] = X[(int64_t)(j + t) * ldx + r];
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+        for (int u = 0; u < CK; ++u)
+          if (i0 + u < k) {
+            const double* ur = U + (int64_t)(i0 + u) * d + j;
 #pragma unroll
-          for (int u = 0; u < CK; ++u)
-            if (i0 + u < k) acc[u] = fma(Us[(i0 + u) * d + j + t], x[t], acc[u]);
+            for (int t = 0; t < 4; ++t) acc[u] = fma(ur[t], x[t], acc[u]);
+          }
       }
       for (; j < d; ++j) {
         const double x = X[(int64_t)j * ldx + r];
 #pragma unroll
         for (int u = 0; u < CK; ++u)
-          if (i0 + u < k) acc[u] = fma(Us[(i0 + u) * d + j], x, acc[u]);
+          if (i0 + u < k) acc[u] = fma(U[(int64_t)(i0 + u) * d + j], x, acc[u]);
       }
 #pragma unroll
       for (int u = 0; u < CK; ++u)
@@ -256,7 +261,10 @@ int sg4_block_combine(sg4_plan* p, int32_t k, int32_t d, int64_t n, const double
   if (xb < yb + ye && yb < xb + xe) { set_error("sg4_block_combine: X and Y overlap"); return SG4_ERR_ARG; }
   const hipStream_t st = (hipStream_t)plan_stream(p);
   const unsigned nb = (unsigned)std::min<int64_t>((n + 255) / 256, 65535);
-  hipLaunchKernelGGL(k_combine, dim3(nb), dim3(256), sizeof(double) * k * d, st, k, d, n, U, X, ldx, beta, Y, ldy);
+  if (k <= CK)
+    hipLaunchKernelGGL(k_combine<CK>, dim3(nb), dim3(256), 0, st, k, d, n, U, X, ldx, beta, Y, ldy);
+  else
+    hipLaunchKernelGGL(k_combine<CKW>, dim3(nb), dim3(256), 0, st, k, d, n, U, X, ldx, beta, Y, ldy);
   BK_TRY(hipGetLastError());
   return SG4_OK;
 }

@@ -79,8 +79,8 @@ class _Dense:
     through the library's block kernels (sg4_block_gram, sg4_block_combine,
     on the operator's plan and stream); larger ones through the split-K
     batched product and the library GEMM, which are faster there (10D L=8:
-    gram 20 x 20 0.28 against 0.30 ms, 16 x 84 0.67 against 0.31 ms;
-    rotation 20 x 84 0.48 against 0.59 ms, 36 x 84 0.83 against 0.66 ms).
+    gram 20 x 20 0.19 against 0.30 ms, 16 x 84 0.67 against 0.31 ms;
+    rotation 20 x 84 0.34 against 0.59 ms, 36 x 84 0.69 against 0.64 ms).
     CPU tensors (host tests of the driver with a dense matrix) use plain
     torch products."""
 
