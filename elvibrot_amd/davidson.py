@@ -209,8 +209,10 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
         m = W.shape[0]
         if ndim + m > max_WP:          # restart on the current Ritz vectors
             keep = min(max(nb_WP + block, nb_WP), max_WP - m)
-            Vb[:keep] = X_all[:keep]
-            HV[:keep] = HX_all[:keep]
+            vk = np.ascontiguousarray(vec[:, :keep].T)
+            Xk, HXk = dn.comb(vk, Vb[:ndim]), dn.comb(vk, HV[:ndim])
+            Vb[:keep] = Xk
+            HV[:keep] = HXk
             Hp[:keep, :keep] = vec[:, :keep].T @ Hp[:ndim, :ndim] @ vec[:, :keep]
             ndim = keep
             W = orthonormalize(new, ndim)
@@ -228,14 +230,13 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
         Hp[ndim:ndim + m, ndim:ndim + m] = 0.5 * (G[ndim:] + G[ndim:].T)
         ndim += m
         ev, vec = np.linalg.eigh(Hp[:ndim, :ndim])
-        # Ritz vectors: the nb_WP wanted ones and those a restart keeps
-        nk = min(ndim, nb_WP + block)
-        vec_t = np.ascontiguousarray(vec[:, :nk].T)
-        X_all = dn.comb(vec_t, Vb[:ndim])
-        HX_all = dn.comb(vec_t, HV[:ndim])
+        # the nb_WP wanted Ritz vectors (those a restart keeps beyond them are
+        # formed at the restart, from vec)
+        vec_t = np.ascontiguousarray(vec[:, :nb_WP].T)
+        X = dn.comb(vec_t, Vb[:ndim])
+        HX = dn.comb(vec_t, HV[:ndim])
         E = ev[:nb_WP]
-        X = X_all[:nb_WP]
-        R = HX_all[:nb_WP] - torch.tensor(E, device=dev)[:, None] * X
+        R = HX - torch.tensor(E, device=dev)[:, None] * X
         rn = np.sqrt(np.maximum(np.diagonal(dn.gram(R, R).cpu().numpy()), 0.0))
         dE = np.abs(E - Ene0) if Ene0 is not None else np.full(nb_WP, np.inf)
         conv = (rn < conv_resi) & (dE < conv_ene)
@@ -255,7 +256,7 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
             # FP64 phase: restart on the Ritz vectors with an FP64 H.psi
             n_low = n_hpsi
             keep = min(ndim, nb_WP + block)
-            Q = orthonormalize(X_all[:keep].clone(), 0)
+            Q = orthonormalize(dn.comb(np.ascontiguousarray(vec[:, :keep].T), Vb[:ndim]), 0)
             keep = Q.shape[0]
             Vb[:keep] = Q
             HV[:keep] = apply(Vb[:keep].contiguous(), PREC_FP64)
@@ -264,12 +265,11 @@ def sub_propagation_Davidson(op, nb_WP, max_it=200, max_WP=None, conv_resi=5e-4,
             G = dn.gram(Vb[:ndim], HV[:ndim]).cpu().numpy()
             Hp[:ndim, :ndim] = 0.5 * (G + G.T)
             ev, vec = np.linalg.eigh(Hp[:ndim, :ndim])
-            vec_t = np.ascontiguousarray(vec.T)
-            X_all = dn.comb(vec_t, Vb[:ndim])
-            HX_all = dn.comb(vec_t, HV[:ndim])
+            vec_t = np.ascontiguousarray(vec[:, :nb_WP].T)
+            X = dn.comb(vec_t, Vb[:ndim])
+            HX = dn.comb(vec_t, HV[:ndim])
             E = ev[:nb_WP]
-            X = X_all[:nb_WP]
-            R = HX_all[:nb_WP] - torch.tensor(E, device=dev)[:, None] * X
+            R = HX - torch.tensor(E, device=dev)[:, None] * X
             rn = np.sqrt(np.maximum(np.diagonal(dn.gram(R, R).cpu().numpy()), 0.0))
             conv = np.zeros(nb_WP, dtype=bool)
             Ene0 = E.copy()
