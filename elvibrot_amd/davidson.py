@@ -75,12 +75,14 @@ def _gram(A, B, chunks=256):
 
 class _Dense:
     """Products over packed vectors (rows of length nb_ba) on the device.
-    Grams with at most 400 outputs and rotations to at most 20 rows go
-    through the library's block kernels (sg4_block_gram, sg4_block_combine,
-    on the operator's plan and stream); larger ones through the split-K
-    batched product and the library GEMM, which are faster there (10D L=8:
-    gram 20 x 20 0.19 against 0.30 ms, 16 x 84 0.67 against 0.31 ms;
-    rotation 20 x 84 0.34 against 0.59 ms, 36 x 84 0.69 against 0.64 ms).
+    Grams with at most 400 outputs or at most 16 columns (up to 124 rows)
+    and rotations to at most 20 rows go through the library's block kernels
+    (sg4_block_gram, sg4_block_combine, on the operator's plan and stream);
+    the others through the split-K batched product and the library GEMM,
+    which are faster there (10D L=8: gram 20 x 20 0.19 against 0.32 ms,
+    100 x 16 0.47 against 0.57 ms, 16 x 100 0.56 against 0.31 ms, 36 x 36
+    0.37 against 0.30 ms; rotation 20 x 84 0.34 against 0.59 ms, 36 x 84
+    0.69 against 0.64 ms).
     CPU tensors (host tests of the driver with a dense matrix) use plain
     torch products."""
 
@@ -99,7 +101,7 @@ class _Dense:
         m, n = A.shape
         q = B.shape[0]
         need = C.c_int64()
-        if m * q > 400 or lib().sg4_block_gram_work(m, q, C.byref(need)) != 0:
+        if (m * q > 400 and q > 16) or lib().sg4_block_gram_work(m, q, C.byref(need)) != 0:
             return _gram(A, B)
         if self.work is None or self.work.numel() < need.value:
             self.work = torch.empty(need.value, device=A.device, dtype=torch.float64)
