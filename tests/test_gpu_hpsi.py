@@ -240,6 +240,21 @@ def test_errors_are_reported(cuda):
     op.apply(x[1:2], x[0:1])
 
 
+def test_mode_beyond_the_matrix_core_tile(cuda):
+    """2D L=16: modes of 33 points, more than a matrix-core tile holds.  The
+    split-precision mode refuses the plan with the reason (before any
+    launch); FP64 and FP32 run it on the grid path, FP64 to the parity bar."""
+    b, op, o, V = hh_pair(2, 16, max_nvec=4)
+    psi = np.random.default_rng(33).standard_normal((b.nb_ba, 3))
+    ref = o.hpsi(psi, V, -0.5 * np.ones(2))
+    assert rel(op.apply_host(psi), ref) < TOL
+    g32 = op.apply_host(psi, E.PREC_F32)
+    assert np.linalg.norm(g32 - ref) / np.linalg.norm(ref) < 2e-6
+    with pytest.raises(E.SG4Error, match="more than 32 grid points"):
+        op.apply_host(psi, E.PREC_F32X2)
+    assert rel(op.apply_host(psi), ref) < TOL          # the plan is still usable
+
+
 @pytest.mark.parametrize("D,L", [(6, 6), (10, 5)])
 def test_f32_mode_accuracy(cuda, D, L):
     """FP32 mode contractions (FP64 gather/scatter) against the FP64 oracle:
